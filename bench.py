@@ -1,0 +1,179 @@
+"""Benchmark of the gpusmoldyn timestep on one MI355X.
+
+  python bench.py [--steps K] [--warmup W] [--config c2|c3|c4|c1]
+
+A step is one full timestep (diffuse + unimolecular reactions + boundaries +
+hash + sort + cell table + pair search + bimolecular reactions + compaction)
+of every particle of the workload, with the particle state resident in HBM.
+`value` is particle-steps/s = (sum over timed steps of the particles entering
+the step) / wall time.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import gpusmoldyn_amd as smd  # noqa: E402
+from gpusmoldyn_amd import configs  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "particle-steps/sec (diffuse+sort+react) 1 GPU, batched vs plain; p99 job latency"
+
+
+def stage_bytes(name, n, S, nb, passes):
+    """Algorithmic bytes of one launch of each stage for n particles, S RNG
+    streams, nb buckets (compulsory reads + writes, each datum once)."""
+    if name == "producer":
+        return 48 * n + 172 * S
+    if name == "sort":
+        return 16 * n + 20 * n * (passes - 1)
+    if name == "first":
+        return 4 * (nb + 1) + 4 * n
+    if name == "gather":
+        return 44 * n
+    if name == "candidates":
+        return 24 * n
+    if name == "resolve":
+        return 4 * n
+    if name == "scatter":
+        return 44 * n
+    return 0
+
+
+def build(cfg):
+    if cfg == "c1":
+        reps = [configs.c1()]
+        steps = configs.C1_STEPS
+        desc = dict(workload="c1: A+B->C, 8192 A + 8192 B, 20 um reflective cube, 1000 steps")
+    elif cfg == "c2":
+        reps = [configs.c2()]
+        steps = configs.C2_STEPS
+        desc = dict(workload="c2: 1,048,576 particles, A+B<->C (kf 5e-3, kr 0.1) + C->0 (0.01/s), periodic 100 um",
+                    particles=1 << 20, streams=131072, grid="2^7 per axis")
+    elif cfg == "c3":
+        reps = [configs.c3()]
+        steps = configs.C3_STEPS
+        desc = dict(workload="c3: 4,194,304 particles, 64 Gaussian blobs, A+B<->C, reflective sphere r=60, "
+                             "absorbing z=0, 200 um", particles=1 << 22, streams=262144, grid="256x256x128 x 0.1 um")
+    elif cfg == "c4":
+        reps = configs.c4()
+        steps = configs.C4_STEPS
+        desc = dict(workload="c4: 512 replicas of c1, n log-uniform 2k-64k, kf log-uniform 1e-4..1e-2",
+                    replicas=len(reps), particles=int(sum(len(r[2]) for r in reps)))
+    else:
+        raise SystemExit(f"unknown config {cfg}")
+    return reps, steps, desc
+
+
+def make_engine(reps, radix_bits=8):
+    e = smd.Engine(0, radix_bits)
+    for spec, pos, types in reps:
+        e.add_replica(spec, pos, types)
+    e.finalize()
+    return e
+
+
+def timed(e, steps):
+    e.sync()
+    ps0 = e.particle_steps()
+    t0 = time.perf_counter()
+    e.step(steps)
+    e.sync()
+    t1 = time.perf_counter()
+    ps1 = e.particle_steps()
+    return ps1 - ps0, t1 - t0
+
+
+def roofline(e, reps, nprof=5):
+    prof = e.profile(nprof)
+    lay = e.layout()
+    n = int(e.counts().sum())
+    S = lay["total_streams"]
+    passes = (lay["key_bits"] + 7) // 8
+    rows = {}
+    for k, us in prof.items():
+        b = stage_bytes(k, n, S, lay["total_buckets"], passes)
+        rows[k] = dict(us=us, bytes=b, gbs=(b / (us * 1e-6) / 1e9) if us > 0 else 0.0)
+    dom = max(rows, key=lambda k: rows[k]["us"])
+    step_us = sum(r["us"] for r in rows.values())
+    step_bytes = sum(r["bytes"] for r in rows.values())
+    ach = rows[dom]["gbs"]
+    rl = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+          "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+          "bytes_per_launch": rows[dom]["bytes"], "avg_us": round(rows[dom]["us"], 2)}
+    step = {"us": round(step_us, 2), "bytes": step_bytes,
+            "gbs": round(step_bytes / (step_us * 1e-6) / 1e9, 1) if step_us > 0 else 0.0,
+            "stages_us": {k: round(v["us"], 2) for k, v in rows.items()}}
+    return rl, step
+
+
+def cpu_baseline(reps, budget_s=20.0):
+    import oracle
+
+    spec, pos, types = reps[0]
+    sim = oracle.sim_for(smd, spec, pos, types, cap=4 * len(types) + 1024)
+    g = smd.gauss_table()
+    done_ps, t_used, k = 0, 0.0, 0
+    while t_used < budget_s and k < 1000:
+        n = sim.n
+        t0 = time.perf_counter()
+        sim.step(g, 1)
+        t_used += time.perf_counter() - t0
+        done_ps += n
+        k += 1
+    return {"value": done_ps / t_used, "unit": "particle-steps/s", "cores": oracle.lib().orc_num_threads(),
+            "kind": "port",
+            "sample": f"CPU oracle (C++ -O3, OpenMP producer/pair search), first {k} steps of the same "
+                      f"workload ({len(types)} particles), {t_used:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="c2")
+    ap.add_argument("--radix-bits", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); replicas and slabs would shard "
+              "across GPUs as described in DESIGN.md, but this build runs on a single MI355X.", file=sys.stderr)
+        sys.exit(2)
+    reps, steps, desc = build(args.config)
+    steps = args.steps if args.steps is not None else steps
+    e = make_engine(reps, args.radix_bits)
+    e.step(args.warmup)
+    ps, secs = timed(e, steps)
+    rl, step = roofline(e, reps)
+    out = {
+        "metric": METRIC,
+        "value": ps / secs,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": secs * 1e3 / max(steps, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": desc,
+        "roofline": rl,
+        "step_roofline": step,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(reps, args.cpu_budget)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

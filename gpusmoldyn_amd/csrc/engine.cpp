@@ -1,0 +1,661 @@
+// Host orchestration of the timestep over an ensemble of replicas (a plain
+// simulation is an ensemble of one).  Replaces ParticleSystem::_initialize
+// (particleSystem.cpp:573-751) and ParticleSystem::update (:829-1001): the
+// particle state stays in HBM, particle counts live on the device, and the
+// host only reads a count back (asynchronously) to bound grid sizes and to
+// grow the buffers before they could overflow.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "smd_engine.h"
+
+namespace smd {
+
+#define HIPCHK(x)                                                                          \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess) {                                                                \
+      set_error(std::string("HIP error '") + hipGetErrorString(e_) + "' in " #x);          \
+      return SMD_ERR_HIP;                                                                  \
+    }                                                                                      \
+  } while (0)
+
+template <typename T>
+static hipError_t dalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  return hipMalloc((void**)p, count * sizeof(T));
+}
+
+template <typename T>
+static void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+static inline uint32_t round64(uint32_t x) { return (x + 63u) & ~63u; }
+
+Engine::Engine(int dev, int rbits) : device(dev), radixBits(rbits == 4 ? 4 : 8) {}
+
+Engine::~Engine() {
+  if (stream) {
+    (void)hipSetDevice(device);
+    (void)hipStreamSynchronize(stream);
+  }
+  free_particle_buffers();
+  dfree(dTables);
+  dfree(dStreamReplica);
+  dfree(dBucketStart);
+  dfree(dRepOff);
+  dfree(dRepCount);
+  dfree(dNewOff);
+  dfree(dNewCount);
+  dfree(dAliveBase);
+  dfree(dBirthBase);
+  dfree(dRepAlive);
+  dfree(dRepDeaths);
+  dfree(dRepBirths);
+  dfree(dCounters);
+  dfree(dParticleSteps);
+  dfree(dMtState);
+  dfree(dMtPar);
+  dfree(dGauss);
+  dfree(dFirst);
+  for (auto& ev : ring) (void)hipEventDestroy(ev.ev);
+  if (pinned) (void)hipHostFree(pinned);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+void Engine::free_particle_buffers() {
+  for (int k = 0; k < 2; ++k) {
+    dfree(dPos[k]);
+    dfree(dType[k]);
+  }
+  dfree(dAddedPos);
+  dfree(dBirthType);
+  dfree(dSortedPT);
+  dfree(dCand);
+  dfree(sb.keys);
+  dfree(sb.vals);
+  dfree(sb.keysAlt);
+  dfree(sb.valsAlt);
+  dfree(sb.counts);
+  dfree(dTileAlive);
+  dfree(dTileBirth);
+  dfree(dTileAlivePre);
+  dfree(dTileBirthPre);
+}
+
+int Engine::add_replica(const smd_sim_spec& spec, const float* pos4, const int* types, uint32_t n) {
+  if (finalized) {
+    set_error("add_replica after finalize");
+    return -SMD_ERR_INVALID;
+  }
+  ReplicaTables T;
+  std::string err;
+  std::vector<smd_rxn_derived> der(spec.num_reactions > 0 ? spec.num_reactions : 1);
+  if (!derive_tables(spec, T, der.data(), err)) {
+    set_error(err);
+    return -SMD_ERR_RXN_SETUP;
+  }
+  for (uint32_t i = 0; i < n; ++i)
+    if (types[i] < 1 || types[i] >= spec.num_types) {
+      set_error("initial particle type out of range");
+      return -SMD_ERR_INVALID;
+    }
+  specs.push_back(spec);
+  tabs.push_back(T);
+  derived.push_back(der);
+  pos0.emplace_back(pos4, pos4 + 4 * (size_t)n);
+  type0.emplace_back(types, types + n);
+  return (int)tabs.size() - 1;
+}
+
+smd_status Engine::alloc_particle_buffers(uint32_t c) {
+  cap = c;
+  for (int k = 0; k < 2; ++k) {
+    HIPCHK(dalloc(&dPos[k], cap));
+    HIPCHK(dalloc(&dType[k], cap));
+  }
+  HIPCHK(dalloc(&dAddedPos, cap));
+  HIPCHK(dalloc(&dBirthType, cap));
+  HIPCHK(dalloc(&dSortedPT, cap));
+  HIPCHK(dalloc(&dCand, cap));
+  HIPCHK(dalloc(&sb.keys, cap));
+  HIPCHK(dalloc(&sb.vals, cap));
+  HIPCHK(dalloc(&sb.keysAlt, cap));
+  HIPCHK(dalloc(&sb.valsAlt, cap));
+  HIPCHK(dalloc(&sb.counts, sort_counts_words(cap, radixBits)));
+  nTilesCap = cap / kCompactTile + 2;
+  HIPCHK(dalloc(&dTileAlive, nTilesCap));
+  HIPCHK(dalloc(&dTileBirth, nTilesCap));
+  HIPCHK(dalloc(&dTileAlivePre, nTilesCap));
+  HIPCHK(dalloc(&dTileBirthPre, nTilesCap));
+  HIPCHK(hipMemsetAsync(dTileAlive, 0, nTilesCap * sizeof(uint32_t), stream));
+  HIPCHK(hipMemsetAsync(dTileBirth, 0, nTilesCap * sizeof(uint32_t), stream));
+  return SMD_OK;
+}
+
+smd_status Engine::finalize(uint32_t capacity) {
+  if (finalized) return SMD_OK;
+  if (tabs.empty()) {
+    set_error("no replicas");
+    return SMD_ERR_INVALID;
+  }
+  HIPCHK(hipSetDevice(device));
+  HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  R = (uint32_t)tabs.size();
+  // Bucket and stream layout.
+  uint64_t buckets = 0, streams = 0, total = 0;
+  anySplit = false;
+  anyBim = false;
+  for (uint32_t r = 0; r < R; ++r) {
+    ReplicaTables& T = tabs[r];
+    T.bucketOff = (uint32_t)buckets;
+    buckets += T.numBuckets;
+    T.streamOff = (uint32_t)streams;
+    streams += round64(T.numStreams);
+    total += type0[r].size();
+    for (int x = 0; x < T.numRxn; ++x)
+      if (specs[r].rxn[x].order == 1 && T.prodY[x] > 0) anySplit = true;
+    if (T.hasBim) anyBim = true;
+  }
+  if (buckets >= (1ull << 31) || streams >= (1ull << 31)) {
+    set_error("too many buckets or streams");
+    return SMD_ERR_INVALID;
+  }
+  deadKey = (uint32_t)buckets;
+  keyBits = 0;
+  while ((1ull << keyBits) <= deadKey) ++keyBits;
+  nb = deadKey + 1;
+  Stot = (uint32_t)streams;
+  uint64_t c = capacity ? capacity : std::max<uint64_t>(anySplit ? 8 * total : total, 1024);
+  if (c < total) c = total;
+  if (anySplit && c < 2 * total) c = 2 * total;
+  if (c >= (1ull << 31)) {
+    set_error("capacity too large");
+    return SMD_ERR_CAPACITY;
+  }
+
+  // Per-replica arrays.
+  HIPCHK(dalloc(&dTables, R));
+  HIPCHK(hipMemcpy(dTables, tabs.data(), R * sizeof(ReplicaTables), hipMemcpyHostToDevice));
+  std::vector<int> streamRep(Stot, -1);
+  for (uint32_t r = 0; r < R; ++r)
+    for (uint32_t k = 0; k < round64(tabs[r].numStreams); ++k) streamRep[tabs[r].streamOff + k] = (int)r;
+  HIPCHK(dalloc(&dStreamReplica, Stot));
+  HIPCHK(hipMemcpy(dStreamReplica, streamRep.data(), Stot * sizeof(int), hipMemcpyHostToDevice));
+  std::vector<uint32_t> bstart(R + 1);
+  for (uint32_t r = 0; r < R; ++r) bstart[r] = tabs[r].bucketOff;
+  bstart[R] = deadKey;
+  HIPCHK(dalloc(&dBucketStart, R + 1));
+  HIPCHK(hipMemcpy(dBucketStart, bstart.data(), (R + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
+  std::vector<uint32_t> off(R + 1), cnt(R);
+  uint64_t o = 0;
+  for (uint32_t r = 0; r < R; ++r) {
+    off[r] = (uint32_t)o;
+    cnt[r] = (uint32_t)type0[r].size();
+    o += cnt[r];
+  }
+  off[R] = (uint32_t)o;
+  for (uint32_t** p : {&dRepOff, &dNewOff}) HIPCHK(dalloc(p, R + 1));
+  for (uint32_t** p : {&dRepCount, &dNewCount, &dAliveBase, &dBirthBase, &dRepAlive, &dRepDeaths, &dRepBirths})
+    HIPCHK(dalloc(p, R));
+  HIPCHK(hipMemcpy(dRepOff, off.data(), (R + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dRepCount, cnt.data(), R * sizeof(uint32_t), hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(dRepDeaths, 0, R * sizeof(uint32_t)));
+  HIPCHK(hipMemset(dRepBirths, 0, R * sizeof(uint32_t)));
+  HIPCHK(dalloc(&dCounters, 8));
+  HIPCHK(dalloc(&dParticleSteps, 1));
+  HIPCHK(hipMemset(dParticleSteps, 0, sizeof(unsigned long long)));
+  dN = dCounters;
+  dNnext = dCounters + 1;
+  dLive = dCounters + 2;
+  const uint32_t tot32 = (uint32_t)total;
+  HIPCHK(hipMemcpy(dN, &tot32, sizeof(uint32_t), hipMemcpyHostToDevice));
+  HIPCHK(dalloc(&dFirst, (size_t)nb + 1));
+
+  // Gaussian table.
+  std::vector<float> g(kGaussDim);
+  gauss_table(g.data(), kGaussDim);
+  HIPCHK(dalloc(&dGauss, kGaussDim));
+  HIPCHK(hipMemcpy(dGauss, g.data(), kGaussDim * sizeof(float), hipMemcpyHostToDevice));
+
+  // Random streams: stream k of a replica uses MersenneTwister.dat record
+  // k % 32768 and seed k of that replica's seed sequence.
+  std::string err;
+  const uint32_t* dat = mt_dat(err);
+  if (!dat) {
+    set_error(err);
+    return SMD_ERR_IO;
+  }
+  std::vector<uint32_t> seeds(Stot, 0u), par(3 * (size_t)Stot, 0u);
+  for (uint32_t r = 0; r < R; ++r) {
+    const ReplicaTables& T = tabs[r];
+    if (!stream_seeds(specs[r].seed, T.numStreams, &seeds[T.streamOff], err)) {
+      set_error(err);
+      return SMD_ERR_IO;
+    }
+    for (uint32_t k = 0; k < T.numStreams; ++k) {
+      const uint32_t rec = k % kMtParamSets;
+      par[T.streamOff + k] = dat[4 * rec];
+      par[(size_t)Stot + T.streamOff + k] = dat[4 * rec + 1];
+      par[2 * (size_t)Stot + T.streamOff + k] = dat[4 * rec + 2];
+    }
+  }
+  HIPCHK(dalloc(&dMtState, (size_t)kMtWords * Stot));
+  HIPCHK(dalloc(&dMtPar, 3 * (size_t)Stot));
+  HIPCHK(hipMemcpy(dMtPar, par.data(), par.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  uint32_t* dSeeds = nullptr;
+  HIPCHK(dalloc(&dSeeds, Stot));
+  HIPCHK(hipMemcpy(dSeeds, seeds.data(), Stot * sizeof(uint32_t), hipMemcpyHostToDevice));
+  HIPCHK(launch_mt_init(dMtState, dSeeds, Stot, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  dfree(dSeeds);
+
+  // Particles.
+  smd_status st = alloc_particle_buffers((uint32_t)c);
+  if (st != SMD_OK) return st;
+  for (uint32_t r = 0; r < R; ++r) {
+    if (cnt[r] == 0) continue;
+    HIPCHK(hipMemcpy(dPos[0] + off[r], pos0[r].data(), cnt[r] * sizeof(float4), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dType[0] + off[r], type0[r].data(), cnt[r] * sizeof(int), hipMemcpyHostToDevice));
+  }
+  cur = 0;
+  Nub = total;
+  knownN = total;
+  stepsDone = 0;
+  HIPCHK(hipHostMalloc((void**)&pinned, kRing * sizeof(uint32_t), hipHostMallocDefault));
+  ring.resize(kRing);
+  for (auto& e : ring) HIPCHK(hipEventCreateWithFlags(&e.ev, hipEventDisableTiming));
+  HIPCHK(hipStreamSynchronize(stream));
+  finalized = true;
+  return SMD_OK;
+}
+
+// Bound on the particle count after all enqueued steps, refreshed from the
+// asynchronous count readbacks that have completed.
+void Engine::refresh_bound(bool block) {
+  while (pendingCount > 0) {
+    Pending& p = ring[pendingHead];
+    if (hipEventQuery(p.ev) != hipSuccess) {
+      if (!block) break;
+      (void)hipEventSynchronize(p.ev);
+    }
+    knownN = pinned[pendingHead];
+    const uint64_t lag = stepsDone - p.step;
+    uint64_t b = knownN;
+    if (anySplit)
+      for (uint64_t k = 0; k < lag && b < (1ull << 40); ++k) b *= 2;
+    Nub = std::min<uint64_t>(Nub, b);
+    pendingHead = (pendingHead + 1) % kRing;
+    --pendingCount;
+  }
+}
+
+smd_status Engine::grow(uint64_t need) {
+  HIPCHK(hipStreamSynchronize(stream));
+  uint64_t nc = std::max<uint64_t>(need, 2ull * cap);
+  if (nc >= (1ull << 31)) {
+    set_error("particle count exceeds the 2^31 capacity limit");
+    return SMD_ERR_CAPACITY;
+  }
+  const uint32_t n = (uint32_t)knownN;
+  float4* keepPos = nullptr;
+  int* keepType = nullptr;
+  HIPCHK(dalloc(&keepPos, n));
+  HIPCHK(dalloc(&keepType, n));
+  if (n) {
+    HIPCHK(hipMemcpy(keepPos, dPos[cur], n * sizeof(float4), hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(keepType, dType[cur], n * sizeof(int), hipMemcpyDeviceToDevice));
+  }
+  free_particle_buffers();
+  smd_status st = alloc_particle_buffers((uint32_t)nc);
+  if (st != SMD_OK) return st;
+  cur = 0;
+  if (n) {
+    HIPCHK(hipMemcpy(dPos[0], keepPos, n * sizeof(float4), hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(dType[0], keepType, n * sizeof(int), hipMemcpyDeviceToDevice));
+  }
+  dfree(keepPos);
+  dfree(keepType);
+  HIPCHK(hipStreamSynchronize(stream));
+  return SMD_OK;
+}
+
+smd_status Engine::ensure_capacity() {
+  refresh_bound(false);
+  const uint64_t needAfter = anySplit ? 2 * Nub : Nub;
+  if (needAfter <= cap) return SMD_OK;
+  refresh_bound(true);
+  const uint64_t need2 = anySplit ? 2 * Nub : Nub;
+  if (need2 <= cap) return SMD_OK;
+  return grow(2 * need2);
+}
+
+smd_status Engine::enqueue_step(StageTimer* timer) {
+  const uint32_t maxN = (uint32_t)std::min<uint64_t>(Nub, cap);
+  auto mark = [&](int stage) {
+    if (timer) (void)hipEventRecord(timer->ev[stage], stream);
+  };
+  mark(0);
+  ProducerArgs pa;
+  pa.tables = dTables;
+  pa.streamReplica = dStreamReplica;
+  pa.repOff = dRepOff;
+  pa.repCount = dRepCount;
+  pa.mtState = dMtState;
+  pa.mtPar = dMtPar;
+  pa.Stot = Stot;
+  pa.gauss = dGauss;
+  pa.pos = dPos[cur];
+  pa.type = dType[cur];
+  pa.addedPos = dAddedPos;
+  pa.birthType = dBirthType;
+  pa.key = sb.keys;
+  pa.deadKey = deadKey;
+  pa.repDeaths = dRepDeaths;
+  pa.repBirths = dRepBirths;
+  pa.tileAlive = dTileAlive;
+  pa.tileBirth = dTileBirth;
+  HIPCHK(launch_producer(pa, stream));
+  mark(1);
+  if (anyBim) {
+    HIPCHK(launch_sort(sb, dN, maxN, keyBits, radixBits, true, stream));
+    mark(2);
+    HIPCHK(launch_first(sb.keys, dN, nb, dFirst, dLive, stream));
+    mark(3);
+    HIPCHK(launch_gather(sb.keys, sb.vals, dPos[cur], dType[cur], dN, maxN, deadKey, dSortedPT, stream));
+    mark(4);
+    CandArgs ca;
+    ca.tables = dTables;
+    ca.bucketStart = dBucketStart;
+    ca.R = (int)R;
+    ca.keys = sb.keys;
+    ca.sortedPT = dSortedPT;
+    ca.first = dFirst;
+    ca.dLive = dLive;
+    ca.cand = dCand;
+    HIPCHK(launch_candidates(ca, maxN, stream));
+    mark(5);
+    ResolveArgs ra;
+    ra.tables = dTables;
+    ra.streamReplica = dStreamReplica;
+    ra.mtState = dMtState;
+    ra.mtPar = dMtPar;
+    ra.Stot = Stot;
+    ra.first = dFirst;
+    ra.cand = dCand;
+    ra.sortedPT = dSortedPT;
+    ra.vals = sb.vals;
+    ra.pos = dPos[cur];
+    ra.type = dType[cur];
+    ra.repDeaths = dRepDeaths;
+    ra.tileAlive = dTileAlive;
+    HIPCHK(launch_resolve(ra, stream));
+    mark(6);
+  } else {
+    for (int s = 2; s <= 6; ++s) mark(s);
+  }
+  FinalizeArgs fa;
+  fa.R = (int)R;
+  fa.repCount = dRepCount;
+  fa.repDeaths = dRepDeaths;
+  fa.repBirths = dRepBirths;
+  fa.newOff = dNewOff;
+  fa.newCount = dNewCount;
+  fa.aliveBase = dAliveBase;
+  fa.birthBase = dBirthBase;
+  fa.repAlive = dRepAlive;
+  fa.dNnext = dNnext;
+  fa.dN = dN;
+  fa.tileAlive = dTileAlive;
+  fa.tileBirth = dTileBirth;
+  fa.tileAlivePre = dTileAlivePre;
+  fa.tileBirthPre = dTileBirthPre;
+  fa.particleSteps = dParticleSteps;
+  HIPCHK(launch_finalize(fa, stream));
+  mark(7);
+  ScatterArgs sa;
+  sa.R = (int)R;
+  sa.dN = dN;
+  sa.repOff = dRepOff;
+  sa.newOff = dNewOff;
+  sa.aliveBase = dAliveBase;
+  sa.birthBase = dBirthBase;
+  sa.repAlive = dRepAlive;
+  sa.tileAlivePre = dTileAlivePre;
+  sa.tileBirthPre = dTileBirthPre;
+  sa.pos = dPos[cur];
+  sa.type = dType[cur];
+  sa.addedPos = dAddedPos;
+  sa.birthType = dBirthType;
+  sa.posOut = dPos[cur ^ 1];
+  sa.typeOut = dType[cur ^ 1];
+  HIPCHK(launch_scatter(sa, maxN, stream));
+  mark(8);
+  cur ^= 1;
+  std::swap(dRepOff, dNewOff);
+  std::swap(dRepCount, dNewCount);
+  std::swap(dN, dNnext);
+  ++stepsDone;
+  if (anySplit) Nub = std::min<uint64_t>(2 * Nub, (uint64_t)cap);
+  // Asynchronous readback of the new total.
+  if (pendingCount == kRing) refresh_bound(true);
+  const int slot = (pendingHead + pendingCount) % kRing;
+  HIPCHK(hipMemcpyAsync(&pinned[slot], dN, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipEventRecord(ring[slot].ev, stream));
+  ring[slot].step = stepsDone;
+  ++pendingCount;
+  return SMD_OK;
+}
+
+smd_status Engine::step(int nsteps) {
+  if (!finalized) {
+    set_error("engine not finalized");
+    return SMD_ERR_INVALID;
+  }
+  if (broken) {
+    set_error("engine state was consumed by a debug stage call");
+    return SMD_ERR_INVALID;
+  }
+  HIPCHK(hipSetDevice(device));
+  for (int s = 0; s < nsteps; ++s) {
+    smd_status st = ensure_capacity();
+    if (st != SMD_OK) return st;
+    st = enqueue_step(nullptr);
+    if (st != SMD_OK) return st;
+  }
+  return SMD_OK;
+}
+
+smd_status Engine::sync() {
+  HIPCHK(hipSetDevice(device));
+  HIPCHK(hipStreamSynchronize(stream));
+  refresh_bound(true);
+  return SMD_OK;
+}
+
+smd_status Engine::particle_steps(unsigned long long* out) {
+  smd_status st = sync();
+  if (st != SMD_OK) return st;
+  HIPCHK(hipMemcpy(out, dParticleSteps, sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  return SMD_OK;
+}
+
+smd_status Engine::counts(uint32_t* out) {
+  smd_status st = sync();
+  if (st != SMD_OK) return st;
+  HIPCHK(hipMemcpy(out, dRepCount, R * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  return SMD_OK;
+}
+
+smd_status Engine::get(int r, float* pos4, int* types, uint32_t capN, uint32_t* nOut) {
+  if (r < 0 || (uint32_t)r >= R) {
+    set_error("replica id out of range");
+    return SMD_ERR_INVALID;
+  }
+  smd_status st = sync();
+  if (st != SMD_OK) return st;
+  uint32_t off = 0, n = 0;
+  HIPCHK(hipMemcpy(&off, dRepOff + r, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(&n, dRepCount + r, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (nOut) *nOut = n;
+  const uint32_t m = std::min(n, capN);
+  if (m && pos4) HIPCHK(hipMemcpy(pos4, dPos[cur] + off, m * sizeof(float4), hipMemcpyDeviceToHost));
+  if (m && types) HIPCHK(hipMemcpy(types, dType[cur] + off, m * sizeof(int), hipMemcpyDeviceToHost));
+  return SMD_OK;
+}
+
+smd_status Engine::get_streams(int r, uint32_t* words, uint32_t capStreams, bool set) {
+  if (r < 0 || (uint32_t)r >= R) {
+    set_error("replica id out of range");
+    return SMD_ERR_INVALID;
+  }
+  smd_status st = sync();
+  if (st != SMD_OK) return st;
+  const ReplicaTables& T = tabs[r];
+  const uint32_t m = std::min(T.numStreams, capStreams);
+  std::vector<uint32_t> buf((size_t)m);
+  for (int w = 0; w < kMtWords; ++w) {
+    uint32_t* dsrc = dMtState + (size_t)w * Stot + T.streamOff;
+    if (set) {
+      for (uint32_t k = 0; k < m; ++k) buf[k] = words[(size_t)k * kMtWords + w];
+      HIPCHK(hipMemcpy(dsrc, buf.data(), m * sizeof(uint32_t), hipMemcpyHostToDevice));
+    } else {
+      HIPCHK(hipMemcpy(buf.data(), dsrc, m * sizeof(uint32_t), hipMemcpyDeviceToHost));
+      for (uint32_t k = 0; k < m; ++k) words[(size_t)k * kMtWords + w] = buf[k];
+    }
+  }
+  return SMD_OK;
+}
+
+smd_status Engine::debug_stage(int stage, float* pos4, int* types, uint32_t* keysSorted, uint32_t* valsSorted,
+                               int* cand, uint32_t* nTotal, uint32_t* nLive) {
+  if (!finalized || broken) {
+    set_error("engine not usable");
+    return SMD_ERR_INVALID;
+  }
+  HIPCHK(hipSetDevice(device));
+  smd_status st = sync();
+  if (st != SMD_OK) return st;
+  uint32_t n = 0;
+  HIPCHK(hipMemcpy(&n, dN, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  const uint32_t maxN = n;
+  broken = true;
+  ProducerArgs pa;
+  pa.tables = dTables;
+  pa.streamReplica = dStreamReplica;
+  pa.repOff = dRepOff;
+  pa.repCount = dRepCount;
+  pa.mtState = dMtState;
+  pa.mtPar = dMtPar;
+  pa.Stot = Stot;
+  pa.gauss = dGauss;
+  pa.pos = dPos[cur];
+  pa.type = dType[cur];
+  pa.addedPos = dAddedPos;
+  pa.birthType = dBirthType;
+  pa.key = sb.keys;
+  pa.deadKey = deadKey;
+  pa.repDeaths = dRepDeaths;
+  pa.repBirths = dRepBirths;
+  pa.tileAlive = dTileAlive;
+  pa.tileBirth = dTileBirth;
+  HIPCHK(launch_producer(pa, stream));
+  uint32_t live = 0;
+  if (stage >= 2) {
+    HIPCHK(launch_sort(sb, dN, maxN, keyBits, radixBits, true, stream));
+    HIPCHK(launch_first(sb.keys, dN, nb, dFirst, dLive, stream));
+    HIPCHK(launch_gather(sb.keys, sb.vals, dPos[cur], dType[cur], dN, maxN, deadKey, dSortedPT, stream));
+  }
+  if (stage >= 3) {
+    CandArgs ca;
+    ca.tables = dTables;
+    ca.bucketStart = dBucketStart;
+    ca.R = (int)R;
+    ca.keys = sb.keys;
+    ca.sortedPT = dSortedPT;
+    ca.first = dFirst;
+    ca.dLive = dLive;
+    ca.cand = dCand;
+    HIPCHK(launch_candidates(ca, maxN, stream));
+  }
+  if (stage >= 4) {
+    ResolveArgs ra;
+    ra.tables = dTables;
+    ra.streamReplica = dStreamReplica;
+    ra.mtState = dMtState;
+    ra.mtPar = dMtPar;
+    ra.Stot = Stot;
+    ra.first = dFirst;
+    ra.cand = dCand;
+    ra.sortedPT = dSortedPT;
+    ra.vals = sb.vals;
+    ra.pos = dPos[cur];
+    ra.type = dType[cur];
+    ra.repDeaths = dRepDeaths;
+    ra.tileAlive = dTileAlive;
+    HIPCHK(launch_resolve(ra, stream));
+  }
+  HIPCHK(hipStreamSynchronize(stream));
+  if (stage >= 2) HIPCHK(hipMemcpy(&live, dLive, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (nTotal) *nTotal = n;
+  if (nLive) *nLive = live;
+  if (n) {
+    if (pos4) HIPCHK(hipMemcpy(pos4, dPos[cur], n * sizeof(float4), hipMemcpyDeviceToHost));
+    if (types) HIPCHK(hipMemcpy(types, dType[cur], n * sizeof(int), hipMemcpyDeviceToHost));
+    if (stage == 1 && keysSorted) HIPCHK(hipMemcpy(keysSorted, sb.keys, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (stage >= 2 && keysSorted)
+      HIPCHK(hipMemcpy(keysSorted, sb.keys, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (stage >= 2 && valsSorted)
+      HIPCHK(hipMemcpy(valsSorted, sb.vals, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (stage >= 3 && cand && live) HIPCHK(hipMemcpy(cand, dCand, live * sizeof(int), hipMemcpyDeviceToHost));
+  }
+  return SMD_OK;
+}
+
+smd_status Engine::profile(int nsteps, double* us, int maxStages, char* names, int namesCap, int* nStages) {
+  static const char* kNames[8] = {"producer", "sort", "first", "gather", "candidates", "resolve", "finalize",
+                                  "scatter"};
+  if (!finalized || broken) {
+    set_error("engine not usable");
+    return SMD_ERR_INVALID;
+  }
+  HIPCHK(hipSetDevice(device));
+  StageTimer tm;
+  for (int k = 0; k < 9; ++k) HIPCHK(hipEventCreate(&tm.ev[k]));
+  std::vector<double> acc(8, 0.0);
+  for (int s = 0; s < nsteps; ++s) {
+    smd_status st = ensure_capacity();
+    if (st != SMD_OK) return st;
+    st = enqueue_step(&tm);
+    if (st != SMD_OK) return st;
+    HIPCHK(hipEventSynchronize(tm.ev[8]));
+    for (int k = 0; k < 8; ++k) {
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, tm.ev[k], tm.ev[k + 1]));
+      acc[k] += ms * 1000.0;
+    }
+  }
+  for (int k = 0; k < 9; ++k) (void)hipEventDestroy(tm.ev[k]);
+  const int m = std::min(8, maxStages);
+  for (int k = 0; k < m; ++k) us[k] = nsteps > 0 ? acc[k] / nsteps : 0.0;
+  if (nStages) *nStages = m;
+  if (names && namesCap > 0) {
+    std::string s;
+    for (int k = 0; k < m; ++k) {
+      s += kNames[k];
+      if (k + 1 < m) s += "\n";
+    }
+    std::strncpy(names, s.c_str(), (size_t)namesCap - 1);
+    names[namesCap - 1] = 0;
+  }
+  return SMD_OK;
+}
+
+}  // namespace smd

@@ -1,0 +1,836 @@
+// HIP kernels of the timestep for gfx950 (wave64).
+//
+// One step over an ensemble of R >= 1 replicas (a plain simulation is R = 1):
+//   k_producer    diffuse + unimolecular reactions + boundaries + hash   (one thread per RNG stream)
+//   k_sort_*      stable LSD radix sort of (bucket key, particle index)
+//   k_first       lower bound of every bucket in the sorted keys
+//   k_gather      sorted copy of position and type
+//   k_candidates  27-neighbour-cell search for each particle's nearest reactive partner
+//   k_resolve     mutual-nearest pairs react (bimReact)                (one thread per RNG stream)
+//   k_finalize    per-replica counts, offsets and compaction-tile prefixes (one workgroup)
+//   k_scatter     order-preserving compaction of survivors, births appended per replica
+#include <hip/hip_runtime.h>
+
+#include "smd_device.h"
+#include "smd_kernels.h"
+
+namespace smd {
+
+constexpr int kBS = 256;  // workgroup size of the per-stream kernels
+
+// ---------------------------------------------------------------------------
+// RNG stream initialisation (MersenneTwister.cuh:40-49 / :80-91) with host
+// supplied seeds.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_mt_init(uint32_t* __restrict__ st, const uint32_t* __restrict__ seeds,
+                                                 uint32_t S) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= S) return;
+  uint32_t v = seeds[g];
+  st[g] = v;
+  for (int i = 1; i < kMtNN; ++i) {
+    v = 1812433253u * (v ^ (v >> 30)) + (uint32_t)i;
+    st[(size_t)i * S + g] = v;
+  }
+  st[(size_t)kMtNN * S + g] = 0u;
+}
+
+// Draws `count` numbers from each stream g < nStreams (out[g * count + k]),
+// advancing the stored state: the stream-level parity hook.
+__global__ __launch_bounds__(kBS) void k_mt_fill(uint32_t* __restrict__ st, const uint32_t* __restrict__ par,
+                                                 uint32_t S, uint32_t nStreams, int count, int batch3,
+                                                 uint32_t* __restrict__ out) {
+  __shared__ uint32_t lds[kMtNN * kBS];
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nStreams) return;
+  Mt m;
+  mt_load<kBS>(m, &lds[threadIdx.x], st, par, S, g);
+  int k = 0;
+  if (batch3) {
+    for (; k + 3 <= count; k += 3) {
+      uint32_t a, b, c;
+      mt_draw3<kBS>(m, a, b, c);
+      out[(size_t)g * count + k] = a;
+      out[(size_t)g * count + k + 1] = b;
+      out[(size_t)g * count + k + 2] = c;
+    }
+  }
+  for (; k < count; ++k) out[(size_t)g * count + k] = mt_draw<kBS>(m);
+  mt_store<kBS>(m, st, S, g);
+}
+
+// ---------------------------------------------------------------------------
+// Producer: diffuse (particles_kernel.cu:125-166) + uniReact
+// (reaction_kernel.cuh:263-342) + boundaries (particles_kernel.cu:60-95 and
+// the surface semantics of surfaces_kernel.cuh) + calcHash
+// (particles_kernel.cu:188-207), fused.  Thread g owns stream g and handles
+// the particles i = s, s + S, s + 2S, ... of its replica in increasing order,
+// so every stream's draws happen in a fixed order.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBS) void k_producer(ProducerArgs a) {
+  __shared__ uint32_t lds_mt[kMtNN * kBS];
+  __shared__ float lds_gauss[kGaussDim];
+  for (int k = threadIdx.x; k < kGaussDim; k += kBS) lds_gauss[k] = a.gauss[k];
+  __syncthreads();
+
+  const uint32_t g = blockIdx.x * kBS + threadIdx.x;
+  const uint32_t lane = threadIdx.x & 63u;
+  const int r = (g < a.Stot) ? a.streamReplica[g] : -1;
+  const int rw = __builtin_amdgcn_readfirstlane(r);  // a wave never spans two replicas
+  uint32_t nDeaths = 0, nBirths = 0;
+
+  if (rw >= 0) {
+    const ReplicaTables& T = a.tables[rw];
+    const uint32_t S = T.numStreams;
+    const uint32_t s = g - T.streamOff;
+    const uint32_t s0 = (g & ~63u) - T.streamOff;  // stream of lane 0
+    const bool laneOK = (r >= 0) && (s < S);
+    const uint32_t n = a.repCount[rw];
+    const uint32_t off = a.repOff[rw];
+    const uint32_t bucketOff = T.bucketOff;
+    Mt m;
+    if (laneOK) mt_load<kBS>(m, &lds_mt[threadIdx.x], a.mtState, a.mtPar, a.Stot, g);
+    const uint32_t iters = (n > s0) ? (n - s0 + S - 1) / S : 0;
+    for (uint32_t it = 0; it < iters; ++it) {
+      const uint32_t il = s + it * S;
+      const bool act = laneOK && il < n;
+      const uint32_t i = off + il;
+      bool alive = false, birth = false;
+      if (act) {
+        float4 p = a.pos[i];
+        int t = a.type[i];
+        uint32_t u0, u1, u2;
+        mt_draw3<kBS>(m, u0, u1, u2);
+        const float rt = T.rate[t > 0 ? t : 0];
+        float np[3];
+        np[0] = fmaf(rt, lds_gauss[u0 & (kGaussDim - 1)], p.x);
+        np[1] = fmaf(rt, lds_gauss[u1 & (kGaussDim - 1)], p.y);
+        np[2] = fmaf(rt, lds_gauss[u2 & (kGaussDim - 1)], p.z);
+        float w = p.w;
+        int bt = 0;
+        float bp[3] = {np[0], np[1], np[2]};
+        float bw = p.w;
+        if (t <= 0) {
+          t = 0;
+        } else if (T.hasUni) {
+          const int cnt = T.uniCount[t];
+          const int base = T.uniStart[t];
+          for (int j = 0; j < cnt; ++j) {
+            const int rx = T.uniList[base + j];
+            if (rand_real(mt_draw<kBS>(m)) < T.prob[rx]) {
+              if (T.prodX[rx] <= 0) {
+                t = 0;
+              } else {
+                t = T.prodX[rx];
+                const int t2 = T.prodY[rx];
+                if (t2 > 0) {
+                  bt = t2;
+                  if (T.unbindOffsets) {
+                    const float th = smd_theta(rand_real(mt_draw<kBS>(m)));
+                    const float ph = rand_real(mt_draw<kBS>(m)) * 6.28318548f;
+                    const float ch = rand_real(mt_draw<kBS>(m)) * 6.28318548f;
+                    float v1[3], v2[3];
+                    rotate_offsets(th, ph, ch, T.prodPos[rx], v1, v2);
+                    bp[0] = np[0] + v2[0];
+                    bp[1] = np[1] + v2[1];
+                    bp[2] = np[2] + v2[2];
+                    np[0] = np[0] + v1[0];
+                    np[1] = np[1] + v1[1];
+                    np[2] = np[2] + v1[2];
+                    w = 0.0f;
+                    bw = 0.0f;
+                  }
+                }
+              }
+              break;
+            }
+          }
+        }
+        uint32_t key = a.deadKey;
+        if (t > 0) {
+          const float o[3] = {p.x, p.y, p.z};
+          if (!apply_box(T, np)) {
+            t = 0;
+          } else {
+            if (T.numSpheres) apply_spheres(T, o, np);
+            key = bucketOff + grid_hash(T, np[0], np[1], np[2]);
+          }
+        }
+        a.pos[i] = make_float4(np[0], np[1], np[2], w);
+        a.type[i] = t;
+        a.key[i] = key;
+        a.birthType[i] = bt;
+        if (bt) a.addedPos[i] = make_float4(bp[0], bp[1], bp[2], bw);
+        alive = t > 0;
+        birth = bt > 0;
+        nDeaths += alive ? 0u : 1u;
+        nBirths += birth ? 1u : 0u;
+      }
+      // Compaction tile counters: lanes hold consecutive particle indices.
+      const uint64_t am = __ballot(alive);
+      const uint64_t bm = __ballot(birth);
+      if (lane == 0) {
+        const uint32_t base = off + s0 + it * S;
+        const uint32_t t0 = base / kCompactTile;
+        const uint32_t split = (t0 + 1) * kCompactTile - base;  // first lane in the next tile
+        const uint64_t loMask = split >= 64 ? ~0ull : ((1ull << split) - 1ull);
+        const uint32_t aLo = __popcll(am & loMask), aHi = __popcll(am & ~loMask);
+        const uint32_t bLo = __popcll(bm & loMask), bHi = __popcll(bm & ~loMask);
+        if (aLo) atomicAdd(&a.tileAlive[t0], aLo);
+        if (aHi) atomicAdd(&a.tileAlive[t0 + 1], aHi);
+        if (bLo) atomicAdd(&a.tileBirth[t0], bLo);
+        if (bHi) atomicAdd(&a.tileBirth[t0 + 1], bHi);
+      }
+    }
+    if (laneOK) mt_store<kBS>(m, a.mtState, a.Stot, g);
+  }
+  nDeaths = wave_sum(nDeaths);
+  nBirths = wave_sum(nBirths);
+  if (lane == 0 && rw >= 0) {
+    if (nDeaths) atomicAdd(&a.repDeaths[rw], nDeaths);
+    if (nBirths) atomicAdd(&a.repBirths[rw], nBirths);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort of (key, value), RADIX_BITS per pass, 4096-element
+// tiles: upsweep (tile digit counts) -> rowscan (per digit across tiles) ->
+// downsweep (wave64 ballot match ranking, LDS staging, scatter).  The
+// result equals std::stable_sort by key of (key, index) pairs, the contract
+// of RadixSort::sort (radixsort.h:168-185, radixsort.cu:1045-1142).
+// ---------------------------------------------------------------------------
+constexpr int kSortItems = 16;
+constexpr int kSortTile = 256 * kSortItems;
+
+template <int BITS>
+__global__ __launch_bounds__(256) void k_sort_up(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ dN,
+                                                 uint32_t* __restrict__ counts, uint32_t stride, int shift) {
+  constexpr int RADIX = 1 << BITS;
+  __shared__ uint32_t hist[RADIX];
+  const uint32_t n = *dN;
+  const uint32_t tile = blockIdx.x;
+  const uint32_t nTiles = (n + kSortTile - 1) / kSortTile;
+  if (tile >= nTiles) return;
+  for (int d = threadIdx.x; d < RADIX; d += 256) hist[d] = 0;
+  __syncthreads();
+  const uint32_t base = tile * kSortTile;
+#pragma unroll 4
+  for (int k = 0; k < kSortItems; ++k) {
+    const uint32_t e = base + k * 256 + threadIdx.x;
+    if (e < n) atomicAdd(&hist[(keys[e] >> shift) & (RADIX - 1)], 1u);
+  }
+  __syncthreads();
+  for (int d = threadIdx.x; d < RADIX; d += 256) counts[(size_t)d * stride + tile] = hist[d];
+}
+
+// Exclusive scan of 256 values held one per thread; returns the exclusive
+// prefix and writes the total to *total (valid after the call).
+__device__ __forceinline__ uint32_t block_exscan256(uint32_t v, uint32_t* wsum, uint32_t* total) {
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  uint32_t pre = 0;
+  for (uint32_t k = 0; k < w; ++k) pre += wsum[k];
+  if (threadIdx.x == 255) *total = pre + x;
+  __syncthreads();
+  return pre + x - v;
+}
+
+template <int BITS>
+__global__ __launch_bounds__(256) void k_sort_rowscan(uint32_t* __restrict__ counts, uint32_t* __restrict__ totals,
+                                                      const uint32_t* __restrict__ dN, uint32_t stride) {
+  __shared__ uint32_t wsum[4];
+  __shared__ uint32_t tot;
+  const uint32_t n = *dN;
+  const uint32_t nTiles = (n + kSortTile - 1) / kSortTile;
+  const uint32_t d = blockIdx.x;
+  uint32_t* row = counts + (size_t)d * stride;
+  uint32_t carry = 0;
+  for (uint32_t b = 0; b < nTiles; b += 256) {
+    const uint32_t e = b + threadIdx.x;
+    const uint32_t v = (e < nTiles) ? row[e] : 0u;
+    const uint32_t ex = block_exscan256(v, wsum, &tot);
+    if (e < nTiles) row[e] = carry + ex;
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) totals[d] = carry;
+}
+
+template <int BITS>
+__global__ __launch_bounds__(256) void k_sort_down(const uint32_t* __restrict__ keysIn,
+                                                   const uint32_t* __restrict__ valsIn,
+                                                   uint32_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut,
+                                                   const uint32_t* __restrict__ counts,
+                                                   const uint32_t* __restrict__ totals,
+                                                   const uint32_t* __restrict__ dN, uint32_t stride, int shift,
+                                                   int identityVals) {
+  constexpr int RADIX = 1 << BITS;
+  constexpr int DPT = (RADIX + 255) / 256;  // digits per thread in the per-digit loops
+  __shared__ uint32_t s_base[RADIX];
+  __shared__ uint32_t s_wcnt[4][RADIX];
+  __shared__ uint32_t s_tstart[RADIX];
+  __shared__ uint32_t s_key[kSortTile];
+  __shared__ uint32_t s_val[kSortTile];
+  __shared__ uint32_t wsum[4];
+  __shared__ uint32_t tot;
+
+  const uint32_t n = *dN;
+  const uint32_t tile = blockIdx.x;
+  const uint32_t nTiles = (n + kSortTile - 1) / kSortTile;
+  if (tile >= nTiles) return;
+  const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+
+  // Global start of each digit for this tile.
+  {
+    uint32_t carry = 0;
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+      const uint32_t d = q * 256 + t;
+      const uint32_t v = (d < (uint32_t)RADIX) ? totals[d] : 0u;
+      const uint32_t ex = block_exscan256(v, wsum, &tot);
+      if (d < (uint32_t)RADIX) s_base[d] = carry + ex + counts[(size_t)d * stride + tile];
+      carry += tot;
+      __syncthreads();
+    }
+  }
+  for (uint32_t d = t; d < (uint32_t)RADIX; d += 256) {
+    s_wcnt[0][d] = 0;
+    s_wcnt[1][d] = 0;
+    s_wcnt[2][d] = 0;
+    s_wcnt[3][d] = 0;
+  }
+  __syncthreads();
+
+  const uint32_t base = tile * kSortTile + w * (64 * kSortItems);
+  uint32_t kk[kSortItems], vv[kSortItems], rk[kSortItems];
+  const uint64_t lt = lanemask_lt();
+#pragma unroll
+  for (int it = 0; it < kSortItems; ++it) {
+    const uint32_t e = base + it * 64 + lane;
+    const bool valid = e < n;
+    const uint32_t k = valid ? keysIn[e] : 0u;
+    kk[it] = k;
+    vv[it] = identityVals ? e : (valid ? valsIn[e] : 0u);
+    const uint32_t d = (k >> shift) & (RADIX - 1);
+    uint64_t mm = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < BITS; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bb = __ballot(bit);
+      mm &= bit ? bb : ~bb;
+    }
+    const uint32_t prev = s_wcnt[w][d];
+    rk[it] = prev + (uint32_t)__popcll(mm & lt);
+    if (valid && (mm & lt) == 0ull) s_wcnt[w][d] = prev + (uint32_t)__popcll(mm);
+  }
+  __syncthreads();
+  // Exclusive scan over waves per digit, and over digits for the tile.
+  {
+    uint32_t carry = 0;
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+      const uint32_t d = q * 256 + t;
+      uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+      if (d < (uint32_t)RADIX) {
+        c0 = s_wcnt[0][d];
+        c1 = s_wcnt[1][d];
+        c2 = s_wcnt[2][d];
+        c3 = s_wcnt[3][d];
+      }
+      const uint32_t sum = c0 + c1 + c2 + c3;
+      const uint32_t ex = block_exscan256(sum, wsum, &tot);
+      if (d < (uint32_t)RADIX) {
+        s_wcnt[0][d] = 0;
+        s_wcnt[1][d] = c0;
+        s_wcnt[2][d] = c0 + c1;
+        s_wcnt[3][d] = c0 + c1 + c2;
+        s_tstart[d] = carry + ex;
+      }
+      carry += tot;
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < kSortItems; ++it) {
+    const uint32_t e = base + it * 64 + lane;
+    if (e < n) {
+      const uint32_t d = (kk[it] >> shift) & (RADIX - 1);
+      const uint32_t lp = s_tstart[d] + s_wcnt[w][d] + rk[it];
+      s_key[lp] = kk[it];
+      s_val[lp] = vv[it];
+    }
+  }
+  __syncthreads();
+  const uint32_t tileN = min((uint32_t)kSortTile, n - tile * kSortTile);
+  for (uint32_t idx = t; idx < tileN; idx += 256) {
+    const uint32_t k = s_key[idx];
+    const uint32_t d = (k >> shift) & (RADIX - 1);
+    const uint32_t dst = s_base[d] + (idx - s_tstart[d]);
+    keysOut[dst] = k;
+    valsOut[dst] = s_val[idx];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// first[h] = lower bound of bucket h in the sorted keys, for h in [0, nb)
+// (nb = deadKey + 1, so first[deadKey] = number of live particles).  This is
+// the start/end table of reorderDataAndFindCellStartD
+// (particles_kernel.cu:209-272) written for every bucket, so no memset is
+// needed and bucket run [a, b] is [first[a], first[b + 1]).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* __restrict__ k, uint32_t lo, uint32_t hi,
+                                                    uint32_t v) {
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (k[mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void k_first(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ dN,
+                                               uint32_t nb, uint32_t* __restrict__ first,
+                                               uint32_t* __restrict__ dLive) {
+  constexpr uint32_t kPer = 16;
+  const uint32_t n = *dN;
+  const uint32_t h0 = (blockIdx.x * 256 + threadIdx.x) * kPer;
+  if (h0 >= nb) return;
+  const uint32_t h1 = min(h0 + kPer, nb);
+  const uint32_t lo = lower_bound_u32(keys, 0, n, h0);
+  const uint32_t hi = lower_bound_u32(keys, lo, n, h1);
+  for (uint32_t h = h0; h < h1; ++h) {
+    const uint32_t f = lower_bound_u32(keys, lo, hi, h);
+    first[h] = f;
+    if (h == nb - 1) *dLive = f;
+  }
+  if (h1 == nb) first[nb] = n;
+}
+
+// sortedPT[j] = (pos[val[j]].xyz, type[val[j]] as float bits) for live j.
+__global__ __launch_bounds__(256) void k_gather(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                                const float4* __restrict__ pos, const int* __restrict__ type,
+                                                const uint32_t* __restrict__ dN, uint32_t deadKey,
+                                                float4* __restrict__ sortedPT) {
+  const uint32_t j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= *dN) return;
+  if (keys[j] >= deadKey) return;
+  const uint32_t i = vals[j];
+  const float4 p = pos[i];
+  sortedPT[j] = make_float4(p.x, p.y, p.z, __int_as_float(type[i]));
+}
+
+// ---------------------------------------------------------------------------
+// Candidate search (collideD/collideCell, particles_kernel.cu:316-411,
+// :443-497): for each live sorted particle j, the nearest particle k != j in
+// the 27 neighbouring buckets with a bimolecular reaction for the type pair
+// and d^2 <= bindRadius^2 (ties: lower k).  Minimum image on periodic axes.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int replica_of_bucket(const uint32_t* __restrict__ bucketStart, int R, uint32_t key) {
+  int lo = 0, hi = R;  // bucketStart[R] = deadKey; find last r with bucketStart[r] <= key
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (bucketStart[mid] <= key) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ void scan_range(const ReplicaTables& T, const float4* __restrict__ spt, uint32_t a,
+                                           uint32_t b, uint32_t j, float px, float py, float pz, int tj,
+                                           int& best, float& bestD2) {
+  for (uint32_t k = a; k < b; ++k) {
+    if (k == j) continue;
+    const float4 q = spt[k];
+    const int tk = __float_as_int(q.w);
+    if (tk <= 0 || tk >= T.numTypes) continue;
+    const int rx = T.pairRxn[tj * kMaxTypes + tk];
+    if (rx < 0) continue;
+    const float dx = min_image(px - q.x, T, 0);
+    const float dy = min_image(py - q.y, T, 1);
+    const float dz = min_image(pz - q.z, T, 2);
+    const float d2 = dot3(dx, dy, dz, dx, dy, dz);
+    if (d2 <= T.bindR2[rx] && (d2 < bestD2 || (d2 == bestD2 && (int)k < best))) {
+      best = (int)k;
+      bestD2 = d2;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_candidates(CandArgs a) {
+  const uint32_t j = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t nLive = *a.dLive;
+  if (j >= nLive) return;
+  const uint32_t key = a.keys[j];
+  const int r = replica_of_bucket(a.bucketStart, a.R, key);
+  const ReplicaTables& T = a.tables[r];
+  int best = -1;
+  if (T.hasBim) {
+    const float4 p = a.sortedPT[j];
+    const int tj = __float_as_int(p.w);
+    float bestD2 = 3.4e38f;
+    const uint32_t local = key - T.bucketOff;
+    const uint32_t bx = T.gbits[0], by = T.gbits[1];
+    const uint32_t mx = (1u << bx) - 1u, my = (1u << by) - 1u, mz = (1u << T.gbits[2]) - 1u;
+    const uint32_t gx = local & mx, gy = (local >> bx) & my, gz = local >> (bx + by);
+    const uint32_t* __restrict__ first = a.first;
+    for (int dz = -1; dz <= 1; ++dz) {
+      const uint32_t zz = (gz + (uint32_t)dz) & mz;
+      for (int dy = -1; dy <= 1; ++dy) {
+        const uint32_t yy = (gy + (uint32_t)dy) & my;
+        const uint32_t row = T.bucketOff + (zz << (bx + by)) + (yy << bx);
+        if (gx >= 1 && gx < mx) {
+          scan_range(T, a.sortedPT, first[row + gx - 1], first[row + gx + 2], j, p.x, p.y, p.z, tj, best, bestD2);
+        } else {
+          for (int dx = -1; dx <= 1; ++dx) {
+            const uint32_t xx = (gx + (uint32_t)dx) & mx;
+            scan_range(T, a.sortedPT, first[row + xx], first[row + xx + 1], j, p.x, p.y, p.z, tj, best, bestD2);
+          }
+        }
+      }
+    }
+  }
+  a.cand[j] = best;
+}
+
+// ---------------------------------------------------------------------------
+// Resolve: thread g owns stream g and walks the sorted particles
+// j = s, s + S, ... of its replica.  A pair (j, c) reacts when each is the
+// other's candidate; the lower sorted index j draws (coin for prob < 1, then
+// theta, phi, chi) and performs bimReact (reaction_kernel.cuh:202-258),
+// writing products into the unsorted arrays at the original indices.
+// Stream state is loaded only by threads that actually draw.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBS) void k_resolve(ResolveArgs a) {
+  __shared__ uint32_t lds_mt[kMtNN * kBS];
+  const uint32_t g = blockIdx.x * kBS + threadIdx.x;
+  if (g >= a.Stot) return;
+  const int r = a.streamReplica[g];
+  if (r < 0) return;
+  const ReplicaTables& T = a.tables[r];
+  if (!T.hasBim) return;
+  const uint32_t S = T.numStreams;
+  const uint32_t s = g - T.streamOff;
+  if (s >= S) return;
+  const uint32_t lo = a.first[T.bucketOff];
+  const uint32_t hi = a.first[T.bucketOff + T.numBuckets];
+  Mt m;
+  bool loaded = false;
+  uint32_t deaths = 0;
+  for (uint32_t j = lo + s; j < hi; j += S) {
+    const int c = a.cand[j];
+    if (c <= (int)j) continue;
+    if (a.cand[c] != (int)j) continue;
+    const float4 p1 = a.sortedPT[j];
+    const float4 p2 = a.sortedPT[c];
+    const int t1 = __float_as_int(p1.w), t2 = __float_as_int(p2.w);
+    const int rx = T.pairRxn[t1 * kMaxTypes + t2];
+    if (!loaded) {
+      mt_load<kBS>(m, &lds_mt[threadIdx.x], a.mtState, a.mtPar, a.Stot, g);
+      loaded = true;
+    }
+    const float pr = T.prob[rx];
+    if (!(pr == 1.0f || rand_real(mt_draw<kBS>(m)) < pr)) continue;
+    const float dc1 = T.D[t1], dc2 = T.D[t2];
+    const float x = (dc1 == 0.0f && dc2 == 0.0f) ? 0.5f : dc2 / (dc1 + dc2);
+    // pos2 taken at its image nearest pos1 on periodic axes
+    const float q0 = p1.x - min_image(p1.x - p2.x, T, 0);
+    const float q1 = p1.y - min_image(p1.y - p2.y, T, 1);
+    const float q2 = p1.z - min_image(p1.z - p2.z, T, 2);
+    const float omx = 1.0f - x;
+    float rp[3] = {fmaf(x, p1.x, omx * q0), fmaf(x, p1.y, omx * q1), fmaf(x, p1.z, omx * q2)};
+    const float th = smd_theta(rand_real(mt_draw<kBS>(m)));
+    const float ph = rand_real(mt_draw<kBS>(m)) * 6.28318548f;
+    const float ch = rand_real(mt_draw<kBS>(m)) * 6.28318548f;
+    float v1[3], v2[3];
+    rotate_offsets(th, ph, ch, T.prodPos[rx], v1, v2);
+    const uint32_t mol1 = a.vals[j], mol2 = a.vals[c];
+    float n1[3] = {rp[0] + v1[0], rp[1] + v1[1], rp[2] + v1[2]};
+    float n2[3] = {rp[0] + v2[0], rp[1] + v2[1], rp[2] + v2[2]};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (T.periodic[d]) {
+        if (n1[d] >= T.hi[d]) n1[d] -= T.L[d];
+        else if (n1[d] < T.lo[d]) n1[d] += T.L[d];
+        if (n2[d] >= T.hi[d]) n2[d] -= T.L[d];
+        else if (n2[d] < T.lo[d]) n2[d] += T.L[d];
+      }
+    }
+    const int nt1 = T.prodX[rx], nt2 = T.prodY[rx];
+    a.pos[mol1] = make_float4(n1[0], n1[1], n1[2], 0.0f);
+    a.pos[mol2] = make_float4(n2[0], n2[1], n2[2], 0.0f);
+    a.type[mol1] = nt1;
+    a.type[mol2] = nt2;
+    if (nt1 <= 0) {
+      ++deaths;
+      atomicSub(&a.tileAlive[mol1 / kCompactTile], 1u);
+    }
+    if (nt2 <= 0) {
+      ++deaths;
+      atomicSub(&a.tileAlive[mol2 / kCompactTile], 1u);
+    }
+  }
+  if (loaded) mt_store<kBS>(m, a.mtState, a.Stot, g);
+  if (deaths) atomicAdd(&a.repDeaths[r], deaths);
+}
+
+// ---------------------------------------------------------------------------
+// Finalize (one workgroup of 1024): per-replica survivor/birth counts and new
+// offsets, and exclusive prefixes of the compaction tile counters (which are
+// reset for the next step).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t block_exscan1024(uint32_t v, uint32_t* wsum, uint32_t* total) {
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  uint32_t pre = 0;
+  for (uint32_t k = 0; k < w; ++k) pre += wsum[k];
+  if (threadIdx.x == 1023) *total = pre + x;
+  __syncthreads();
+  return pre + x - v;
+}
+
+__global__ __launch_bounds__(1024) void k_finalize(FinalizeArgs a) {
+  __shared__ uint32_t wsum[16];
+  __shared__ uint32_t tot;
+  // replicas
+  uint32_t cN = 0, cA = 0, cB = 0;
+  for (uint32_t b = 0; b < (uint32_t)a.R; b += 1024) {
+    const uint32_t r = b + threadIdx.x;
+    uint32_t alive = 0, births = 0;
+    if (r < (uint32_t)a.R) {
+      alive = a.repCount[r] - a.repDeaths[r];
+      births = a.repBirths[r];
+    }
+    const uint32_t exN = block_exscan1024(alive + births, wsum, &tot);
+    const uint32_t tN = tot;
+    __syncthreads();
+    const uint32_t exA = block_exscan1024(alive, wsum, &tot);
+    const uint32_t tA = tot;
+    __syncthreads();
+    const uint32_t exB = block_exscan1024(births, wsum, &tot);
+    const uint32_t tB = tot;
+    if (r < (uint32_t)a.R) {
+      a.newOff[r] = cN + exN;
+      a.newCount[r] = alive + births;
+      a.aliveBase[r] = cA + exA;
+      a.birthBase[r] = cB + exB;
+      a.repAlive[r] = alive;
+      a.repDeaths[r] = 0;
+      a.repBirths[r] = 0;
+    }
+    cN += tN;
+    cA += tA;
+    cB += tB;
+    __syncthreads();
+  }
+  const uint32_t n = *a.dN;
+  if (threadIdx.x == 0) {
+    a.newOff[a.R] = cN;
+    *a.dNnext = cN;
+    *a.particleSteps += n;
+  }
+  // compaction tiles
+  const uint32_t nTiles = (n + kCompactTile - 1) / kCompactTile;
+  uint32_t carA = 0, carB = 0;
+  for (uint32_t b = 0; b < nTiles; b += 1024) {
+    const uint32_t e = b + threadIdx.x;
+    uint32_t va = 0, vb = 0;
+    if (e < nTiles) {
+      va = a.tileAlive[e];
+      vb = a.tileBirth[e];
+      a.tileAlive[e] = 0;
+      a.tileBirth[e] = 0;
+    }
+    const uint32_t exA = block_exscan1024(va, wsum, &tot);
+    const uint32_t tA = tot;
+    __syncthreads();
+    const uint32_t exB = block_exscan1024(vb, wsum, &tot);
+    const uint32_t tB = tot;
+    if (e < nTiles) {
+      a.tileAlivePre[e] = carA + exA;
+      a.tileBirthPre[e] = carB + exB;
+    }
+    carA += tA;
+    carB += tB;
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Scatter: order-preserving compaction (the intent of compactPosAndTypeD,
+// particles_kernel.cu:97-123, done out of place).  Replica r's new segment is
+// [survivors of r in index order, births of r in parent index order].
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
+  __shared__ uint32_t s_wa[16], s_wb[16];
+  const uint32_t n = *a.dN;
+  const uint32_t tile = blockIdx.x;
+  if (tile * kCompactTile >= n) return;
+  const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+  const uint64_t lt = lanemask_lt();
+  constexpr int kItems = kCompactTile / 256;  // 4 chunks of 64 per wave
+  bool al[kItems], bi[kItems];
+  uint32_t ra[kItems], rb[kItems];
+  uint32_t ca = 0, cb = 0;
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    const uint32_t e = tile * kCompactTile + w * 256 + it * 64 + lane;
+    const bool v = e < n;
+    al[it] = v && a.type[e] > 0;
+    bi[it] = v && a.birthType[e] > 0;
+    const uint64_t am = __ballot(al[it]);
+    const uint64_t bm = __ballot(bi[it]);
+    ra[it] = ca + (uint32_t)__popcll(am & lt);
+    rb[it] = cb + (uint32_t)__popcll(bm & lt);
+    ca += (uint32_t)__popcll(am);
+    cb += (uint32_t)__popcll(bm);
+  }
+  if (lane == 0) {
+    s_wa[w] = ca;
+    s_wb[w] = cb;
+  }
+  __syncthreads();
+  uint32_t pa = a.tileAlivePre[tile], pb = a.tileBirthPre[tile];
+  for (uint32_t k = 0; k < w; ++k) {
+    pa += s_wa[k];
+    pb += s_wb[k];
+  }
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    const uint32_t e = tile * kCompactTile + w * 256 + it * 64 + lane;
+    if (!(al[it] || bi[it])) continue;
+    // replica of element e: last r with repOff[r] <= e
+    int lo = 0, hi = a.R;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (a.repOff[mid] <= e) lo = mid;
+      else hi = mid;
+    }
+    const int r = lo;
+    if (al[it]) {
+      const uint32_t dst = a.newOff[r] + (pa + ra[it] - a.aliveBase[r]);
+      a.posOut[dst] = a.pos[e];
+      a.typeOut[dst] = a.type[e];
+    }
+    if (bi[it]) {
+      const uint32_t dst = a.newOff[r] + a.repAlive[r] + (pb + rb[it] - a.birthBase[r]);
+      a.posOut[dst] = a.addedPos[e];
+      a.typeOut[dst] = a.birthType[e];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Launch wrappers.
+// ---------------------------------------------------------------------------
+static inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+hipError_t launch_mt_init(uint32_t* st, const uint32_t* seeds, uint32_t S, hipStream_t stream) {
+  if (S == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mt_init, dim3(cdiv(S, 256)), dim3(256), 0, stream, st, seeds, S);
+  return hipGetLastError();
+}
+
+hipError_t launch_mt_fill(uint32_t* st, const uint32_t* par, uint32_t S, uint32_t nStreams, int count, int batch3,
+                          uint32_t* out, hipStream_t stream) {
+  if (nStreams == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mt_fill, dim3(cdiv(nStreams, kBS)), dim3(kBS), 0, stream, st, par, S, nStreams, count,
+                     batch3, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_producer(const ProducerArgs& a, hipStream_t stream) {
+  hipLaunchKernelGGL(k_producer, dim3(cdiv(a.Stot, kBS)), dim3(kBS), 0, stream, a);
+  return hipGetLastError();
+}
+
+int sort_passes(uint32_t keyBits, int radixBits) { return (int)((keyBits + radixBits - 1) / radixBits); }
+
+size_t sort_counts_words(uint32_t maxN, int radixBits) {
+  return (size_t)(1u << radixBits) * (cdiv(maxN, kSortTile) + 1) + (1u << radixBits);
+}
+
+hipError_t launch_sort(SortBuffers& b, const uint32_t* dN, uint32_t maxN, uint32_t keyBits, int radixBits,
+                       bool identityVals, hipStream_t stream) {
+  const uint32_t maxTiles = cdiv(maxN, kSortTile);
+  if (maxTiles == 0) return hipSuccess;
+  const uint32_t stride = maxTiles + 1;
+  const int radix = 1 << radixBits;
+  uint32_t* totals = b.counts + (size_t)radix * stride;
+  const int passes = sort_passes(keyBits, radixBits);
+  for (int p = 0; p < passes; ++p) {
+    const int shift = p * radixBits;
+    const bool idv = identityVals && p == 0;
+    if (radixBits == 8) {
+      hipLaunchKernelGGL(k_sort_up<8>, dim3(maxTiles), dim3(256), 0, stream, b.keys, dN, b.counts, stride, shift);
+      hipLaunchKernelGGL(k_sort_rowscan<8>, dim3(radix), dim3(256), 0, stream, b.counts, totals, dN, stride);
+      hipLaunchKernelGGL(k_sort_down<8>, dim3(maxTiles), dim3(256), 0, stream, b.keys, b.vals, b.keysAlt,
+                         b.valsAlt, b.counts, totals, dN, stride, shift, idv ? 1 : 0);
+    } else if (radixBits == 4) {
+      hipLaunchKernelGGL(k_sort_up<4>, dim3(maxTiles), dim3(256), 0, stream, b.keys, dN, b.counts, stride, shift);
+      hipLaunchKernelGGL(k_sort_rowscan<4>, dim3(radix), dim3(256), 0, stream, b.counts, totals, dN, stride);
+      hipLaunchKernelGGL(k_sort_down<4>, dim3(maxTiles), dim3(256), 0, stream, b.keys, b.vals, b.keysAlt,
+                         b.valsAlt, b.counts, totals, dN, stride, shift, idv ? 1 : 0);
+    } else {
+      return hipErrorInvalidValue;
+    }
+    uint32_t* tk = b.keys;
+    b.keys = b.keysAlt;
+    b.keysAlt = tk;
+    uint32_t* tv = b.vals;
+    b.vals = b.valsAlt;
+    b.valsAlt = tv;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_first(const uint32_t* keys, const uint32_t* dN, uint32_t nb, uint32_t* first, uint32_t* dLive,
+                        hipStream_t stream) {
+  hipLaunchKernelGGL(k_first, dim3(cdiv(cdiv(nb, 16), 256)), dim3(256), 0, stream, keys, dN, nb, first, dLive);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather(const uint32_t* keys, const uint32_t* vals, const float4* pos, const int* type,
+                         const uint32_t* dN, uint32_t maxN, uint32_t deadKey, float4* sortedPT, hipStream_t stream) {
+  if (maxN == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather, dim3(cdiv(maxN, 256)), dim3(256), 0, stream, keys, vals, pos, type, dN, deadKey,
+                     sortedPT);
+  return hipGetLastError();
+}
+
+hipError_t launch_candidates(const CandArgs& a, uint32_t maxN, hipStream_t stream) {
+  if (maxN == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_candidates, dim3(cdiv(maxN, 256)), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_resolve(const ResolveArgs& a, hipStream_t stream) {
+  hipLaunchKernelGGL(k_resolve, dim3(cdiv(a.Stot, kBS)), dim3(kBS), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize(const FinalizeArgs& a, hipStream_t stream) {
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_scatter(const ScatterArgs& a, uint32_t maxN, hipStream_t stream) {
+  if (maxN == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scatter, dim3(cdiv(maxN, kCompactTile)), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
+}  // namespace smd

@@ -1,0 +1,89 @@
+// The ensemble engine behind the smd_engine C handle.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "smd_kernels.h"
+#include "smd_setup.h"
+
+namespace smd {
+
+struct StageTimer {
+  hipEvent_t ev[9];
+};
+
+class Engine {
+ public:
+  Engine(int device, int radixBits);
+  ~Engine();
+  int add_replica(const smd_sim_spec& spec, const float* pos4, const int* types, uint32_t n);
+  smd_status finalize(uint32_t capacity);
+  smd_status step(int nsteps);
+  smd_status sync();
+  smd_status counts(uint32_t* out);
+  smd_status get(int r, float* pos4, int* types, uint32_t capN, uint32_t* nOut);
+  smd_status get_streams(int r, uint32_t* words, uint32_t capStreams, bool set);
+  smd_status debug_stage(int stage, float* pos4, int* types, uint32_t* keysSorted, uint32_t* valsSorted, int* cand,
+                         uint32_t* nTotal, uint32_t* nLive);
+  smd_status profile(int nsteps, double* us, int maxStages, char* names, int namesCap, int* nStages);
+  smd_status particle_steps(unsigned long long* out);
+
+  int device;
+  int radixBits;
+  bool finalized = false;
+  bool broken = false;
+  hipStream_t stream = nullptr;
+  std::vector<smd_sim_spec> specs;
+  std::vector<ReplicaTables> tabs;
+  std::vector<std::vector<smd_rxn_derived>> derived;
+  std::vector<std::vector<float>> pos0;
+  std::vector<std::vector<int>> type0;
+  uint32_t R = 0, Stot = 0, deadKey = 0, keyBits = 0, nb = 0, cap = 0, nTilesCap = 0;
+  bool anySplit = false, anyBim = false;
+  uint64_t Nub = 0, knownN = 0, stepsDone = 0;
+  int cur = 0;
+
+  // device state
+  ReplicaTables* dTables = nullptr;
+  int* dStreamReplica = nullptr;
+  uint32_t* dBucketStart = nullptr;
+  uint32_t *dRepOff = nullptr, *dRepCount = nullptr, *dNewOff = nullptr, *dNewCount = nullptr;
+  uint32_t *dAliveBase = nullptr, *dBirthBase = nullptr, *dRepAlive = nullptr;
+  uint32_t *dRepDeaths = nullptr, *dRepBirths = nullptr;
+  uint32_t* dCounters = nullptr;
+  unsigned long long* dParticleSteps = nullptr;  // running sum of particles entering each step
+  uint32_t *dN = nullptr, *dNnext = nullptr, *dLive = nullptr;
+  uint32_t *dMtState = nullptr, *dMtPar = nullptr;
+  float* dGauss = nullptr;
+  uint32_t* dFirst = nullptr;
+  float4* dPos[2] = {nullptr, nullptr};
+  int* dType[2] = {nullptr, nullptr};
+  float4* dAddedPos = nullptr;
+  int* dBirthType = nullptr;
+  float4* dSortedPT = nullptr;
+  int* dCand = nullptr;
+  SortBuffers sb = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  uint32_t *dTileAlive = nullptr, *dTileBirth = nullptr, *dTileAlivePre = nullptr, *dTileBirthPre = nullptr;
+
+  // asynchronous count readback
+  struct Pending {
+    hipEvent_t ev = nullptr;
+    uint64_t step = 0;
+  };
+  static constexpr int kRing = 8;
+  std::vector<Pending> ring;
+  uint32_t* pinned = nullptr;
+  int pendingHead = 0, pendingCount = 0;
+
+ private:
+  smd_status alloc_particle_buffers(uint32_t c);
+  void free_particle_buffers();
+  void refresh_bound(bool block);
+  smd_status grow(uint64_t need);
+  smd_status ensure_capacity();
+  smd_status enqueue_step(StageTimer* timer);
+};
+
+}  // namespace smd

@@ -1,0 +1,120 @@
+// Kernel argument blocks and launch wrappers (host-visible).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "smd_types.h"
+
+namespace smd {
+
+struct ProducerArgs {
+  const ReplicaTables* tables;
+  const int* streamReplica;   // per global stream: replica id (padding lanes of a replica included), -1 none
+  const uint32_t* repOff;     // per replica: first particle
+  const uint32_t* repCount;   // per replica: particle count
+  uint32_t* mtState;          // [20][Stot]
+  const uint32_t* mtPar;      // [3][Stot]
+  uint32_t Stot;
+  const float* gauss;         // 4096-entry table
+  float4* pos;                // canonical arrays, updated in place
+  int* type;
+  float4* addedPos;
+  int* birthType;
+  uint32_t* key;
+  uint32_t deadKey;
+  uint32_t* repDeaths;
+  uint32_t* repBirths;
+  uint32_t* tileAlive;
+  uint32_t* tileBirth;
+};
+
+struct SortBuffers {
+  uint32_t* keys;
+  uint32_t* vals;
+  uint32_t* keysAlt;
+  uint32_t* valsAlt;
+  uint32_t* counts;  // sort_counts_words(maxN, radixBits)
+};
+
+struct CandArgs {
+  const ReplicaTables* tables;
+  const uint32_t* bucketStart;  // per replica bucketOff, [R] = deadKey
+  int R;
+  const uint32_t* keys;         // sorted
+  const float4* sortedPT;
+  const uint32_t* first;
+  const uint32_t* dLive;
+  int* cand;
+};
+
+struct ResolveArgs {
+  const ReplicaTables* tables;
+  const int* streamReplica;
+  uint32_t* mtState;
+  const uint32_t* mtPar;
+  uint32_t Stot;
+  const uint32_t* first;
+  const int* cand;
+  const float4* sortedPT;
+  const uint32_t* vals;
+  float4* pos;
+  int* type;
+  uint32_t* repDeaths;
+  uint32_t* tileAlive;
+};
+
+struct FinalizeArgs {
+  int R;
+  const uint32_t* repCount;
+  uint32_t* repDeaths;
+  uint32_t* repBirths;
+  uint32_t* newOff;     // [R+1]
+  uint32_t* newCount;
+  uint32_t* aliveBase;
+  uint32_t* birthBase;
+  uint32_t* repAlive;
+  uint32_t* dNnext;
+  const uint32_t* dN;
+  uint32_t* tileAlive;
+  uint32_t* tileBirth;
+  uint32_t* tileAlivePre;
+  uint32_t* tileBirthPre;
+  unsigned long long* particleSteps;  // += particles entering the step
+};
+
+struct ScatterArgs {
+  int R;
+  const uint32_t* dN;
+  const uint32_t* repOff;
+  const uint32_t* newOff;
+  const uint32_t* aliveBase;
+  const uint32_t* birthBase;
+  const uint32_t* repAlive;
+  const uint32_t* tileAlivePre;
+  const uint32_t* tileBirthPre;
+  const float4* pos;
+  const int* type;
+  const float4* addedPos;
+  const int* birthType;
+  float4* posOut;
+  int* typeOut;
+};
+
+hipError_t launch_mt_init(uint32_t* st, const uint32_t* seeds, uint32_t S, hipStream_t stream);
+hipError_t launch_mt_fill(uint32_t* st, const uint32_t* par, uint32_t S, uint32_t nStreams, int count, int batch3,
+                          uint32_t* out, hipStream_t stream);
+hipError_t launch_producer(const ProducerArgs& a, hipStream_t stream);
+int sort_passes(uint32_t keyBits, int radixBits);
+size_t sort_counts_words(uint32_t maxN, int radixBits);
+hipError_t launch_sort(SortBuffers& b, const uint32_t* dN, uint32_t maxN, uint32_t keyBits, int radixBits,
+                       bool identityVals, hipStream_t stream);
+hipError_t launch_first(const uint32_t* keys, const uint32_t* dN, uint32_t nb, uint32_t* first, uint32_t* dLive,
+                        hipStream_t stream);
+hipError_t launch_gather(const uint32_t* keys, const uint32_t* vals, const float4* pos, const int* type,
+                         const uint32_t* dN, uint32_t maxN, uint32_t deadKey, float4* sortedPT, hipStream_t stream);
+hipError_t launch_candidates(const CandArgs& a, uint32_t maxN, hipStream_t stream);
+hipError_t launch_resolve(const ResolveArgs& a, hipStream_t stream);
+hipError_t launch_finalize(const FinalizeArgs& a, hipStream_t stream);
+hipError_t launch_scatter(const ScatterArgs& a, uint32_t maxN, hipStream_t stream);
+
+}  // namespace smd

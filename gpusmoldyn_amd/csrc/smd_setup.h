@@ -1,0 +1,39 @@
+// Host-side setup routines (setup.cpp) and shared host helpers.
+#pragma once
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/gpusmoldyn.h"
+#include "smd_types.h"
+
+namespace smd {
+
+void gauss_table(float* out, int n);
+double numrxnrate(double step, double a, double b);
+double actrxnrate(double step, double a);
+double bindingradius(double rate, double dt, double difc, double b, int rel);
+double unbindingradius(double pgem, double dt, double difc, double a);
+
+// Builds the device tables of one replica from its spec (fillReactionOrderInfo,
+// rxnsetrate, rxnsetproduct, molSetTimestep, grid and geometry checks).
+// `der` (optional) receives num_reactions derived records.
+bool derive_tables(const smd_sim_spec& sp, ReplicaTables& T, smd_rxn_derived* der, std::string& err);
+
+void set_mt_dat_path(const char* p);
+const uint32_t* mt_dat(std::string& err);  // 4 words per record
+std::string default_data_dir();
+
+struct HostMt {
+  uint32_t st[kMtNN];
+  int idx;
+  uint32_t a, mb, mc;
+  HostMt(uint32_t a, uint32_t mb, uint32_t mc, uint32_t seed);
+  uint32_t next();
+};
+
+bool stream_seeds(uint32_t seed, uint32_t count, uint32_t* out, std::string& err);
+
+void set_error(const std::string& msg);
+
+}  // namespace smd

@@ -1,0 +1,196 @@
+/*
+ * gpusmoldyn_amd: C-ABI of the MI355X (gfx950) Brownian-dynamics
+ * reaction-diffusion timestep.
+ *
+ * Two groups of entry points live here:
+ *
+ *  1. The simulation API (smd_*): a ParticleSystem-style object over one or
+ *     more independent replicas whose particle state stays resident in HBM.
+ *     It replaces ParticleSystem (particleSystem.h:50-218): construction
+ *     (_initialize, particleSystem.cpp:573-751), update() (:829-1001),
+ *     getArray()/setArray() (:1031-1080) and the reaction-table setup of
+ *     smolreact.cpp:173-483 and rxnparam.cpp:42-308.
+ *
+ *  2. The reference's own extern "C" layer (particleSystem.cuh:29-110,
+ *     MersenneTwister.h:49, radixsort.h:45-95), with HIP types, declared in
+ *     gpusmoldyn_legacy.h.
+ *
+ * Every function returns an smd_status (0 = success) unless stated
+ * otherwise; smd_last_error() gives a message for the last failure on the
+ * calling thread.  No function aborts the process.
+ */
+#ifndef GPUSMOLDYN_H_INCLUDED
+#define GPUSMOLDYN_H_INCLUDED
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SMD_MAX_TYPES 16
+#define SMD_MAX_RXN 32
+#define SMD_MAX_SPHERES 4
+#define SMD_MT_PARAM_SETS 32768
+#define SMD_GAUSS_DIM 4096
+
+typedef enum smd_status {
+  SMD_OK = 0,
+  SMD_ERR_INVALID = 1,      /* bad argument or inconsistent specification */
+  SMD_ERR_HIP = 2,          /* HIP runtime error */
+  SMD_ERR_NO_DEVICE = 3,    /* no GPU visible */
+  SMD_ERR_IO = 4,           /* MersenneTwister.dat not found or short */
+  SMD_ERR_CAPACITY = 5,     /* particle count exceeds the allowed maximum */
+  SMD_ERR_EXTINCT = 6,      /* no particles left (update() returns false) */
+  SMD_ERR_UNSUPPORTED = 7,  /* OpenGL interop entry points */
+  SMD_ERR_RXN_SETUP = 8     /* rxnsetrate/rxnsetproduct error code, see smd_last_error */
+} smd_status;
+
+/* Boundary behaviour of a box face. */
+typedef enum smd_boundary {
+  SMD_BC_CLAMP = 0,    /* adjustPositions (particles_kernel.cu:60-95): clamp to [lo + r, hi - r] */
+  SMD_BC_REFLECT = 1,
+  SMD_BC_ABSORB = 2,
+  SMD_BC_PERIODIC = 3
+} smd_boundary;
+
+/* One reaction (the fields of struct Reaction, reaction.h:36-48, that are
+ * inputs; prob/bindRadius/unbindRadius/productPositions are derived). */
+typedef struct smd_reaction {
+  int order;          /* 0, 1 or 2 */
+  int reactants[2];   /* species ids >= 1; -1 when unused */
+  int products[2];    /* species ids >= 1; -1 when unused */
+  int nprod;          /* 0, 1 or 2 */
+  float rate;         /* order 0: molecules/volume/time, 1: 1/time, 2: volume/time */
+  float pgem;         /* geminate probability for the unbinding radius of a
+                         reversible split; <= 0 selects 0.2 (smolreact.cpp:375) */
+  float prob;         /* order 2: reaction probability at contact; <= 0 selects 1
+                         (smolreact.cpp:382-383) */
+} smd_reaction;
+
+/* Everything that defines one simulation (one replica). */
+typedef struct smd_sim_spec {
+  int num_types;                 /* species + the dummy type 0 */
+  float D[SMD_MAX_TYPES];        /* diffusion coefficients, D[0] = 0 */
+  float dt;
+  float volume;                  /* system volume (zeroth-order reactions) */
+  int num_reactions;
+  smd_reaction rxn[SMD_MAX_RXN];
+  float box_lo[3], box_hi[3];
+  int face[6];                   /* smd_boundary per face: -x,+x,-y,+y,-z,+z */
+  float particle_radius;         /* clamp margin (particleSystem.cpp:69) */
+  int num_spheres;
+  float sphere[SMD_MAX_SPHERES][4];  /* centre x,y,z and radius; reflective from both sides */
+  int unbind_offsets;            /* 1: order-1 splits place products at the rotated
+                                    unbinding offsets; 0: both at the reaction point
+                                    (reaction_kernel.cuh:320-321) */
+  unsigned grid_bits[3];         /* log2 of the hash grid dimension per axis (>= 2) */
+  float cell_size[3];            /* <= 0: (box_hi - box_lo) / 2^grid_bits */
+  float origin[3];               /* hash grid origin (usually box_lo) */
+  unsigned num_streams;          /* S: particle i draws from stream i mod S (0 -> 32768) */
+  unsigned seed;                 /* master seed of the stream seeds (4357) */
+} smd_sim_spec;
+
+/* Derived per-reaction parameters (what rxnsetrate/rxnsetproduct compute). */
+typedef struct smd_rxn_derived {
+  float prob;
+  float bind_radius2;      /* bindRadiusSquared (reaction.h:39) */
+  float bind_radius;       /* sqrt of bindRadiusSquared */
+  float unbind_radius;
+  float product_pos[6];
+  int setup_code;          /* rxnsetrate/rxnsetproduct return code (0 ok) */
+} smd_rxn_derived;
+
+typedef struct smd_engine smd_engine;
+
+const char* smd_last_error(void);
+int smd_version(void);
+
+/* Path of MersenneTwister.dat (32768 16-byte records).  Default: the
+ * package's data directory, or $SMD_MT_DAT. */
+smd_status smd_set_mt_dat_path(const char* path);
+
+/* ---- host-side setup (no GPU needed) ---- */
+
+/* createGaussTable (stats.cpp:277-282): 4096 standard-normal quantiles,
+ * mirrored, then passed through the reference's shuffle. */
+smd_status smd_gauss_table(float* out, int n);
+/* rxnparam.cpp:42-308 (FP64). */
+double smd_numrxnrate(double step, double a, double b);
+double smd_actrxnrate(double step, double a);
+double smd_bindingradius(double rate, double dt, double difc, double b, int rel);
+double smd_unbindingradius(double pgem, double dt, double difc, double a);
+/* Derived reaction parameters of a spec (smolreact.cpp:173-483). */
+smd_status smd_spec_derive(const smd_sim_spec* spec, smd_rxn_derived* out);
+/* Hash-grid cell size and origin the engine uses for a spec (periodic axes
+ * get a cell size that tiles the box exactly). */
+smd_status smd_spec_cells(const smd_sim_spec* spec, float* cell3, float* origin3);
+/* Seeds of S streams: output k of the dcmt stream with parameter set 0
+ * seeded with `seed` (MersenneTwister.dat record 0). */
+smd_status smd_stream_seeds(unsigned seed, unsigned count, uint32_t* out);
+/* The dcmt parameters (matrix_a, mask_b, mask_c) of record `index`. */
+smd_status smd_mt_params(unsigned index, uint32_t* abc);
+
+/* ---- simulation engine ---- */
+
+smd_status smd_device_count(int* n);
+/* Create an engine on GPU `device`.  radix_bits: 4 or 8 (sort digit width),
+ * 0 = default (8). */
+smd_engine* smd_engine_create(int device, int radix_bits);
+void smd_engine_destroy(smd_engine* e);
+/* Add a replica with n initial particles (pos: n float4, types: n ints).
+ * Returns the replica id (>= 0) or -smd_status. */
+int smd_engine_add_replica(smd_engine* e, const smd_sim_spec* spec, const float* pos4, const int* types, unsigned n);
+/* Allocate device memory for `capacity` particles in total (0: automatic),
+ * upload, initialise the random streams. */
+smd_status smd_engine_finalize(smd_engine* e, unsigned capacity);
+/* Advance every replica by `nsteps` timesteps.  Asynchronous with respect to
+ * the host except when capacity must be checked. */
+smd_status smd_engine_step(smd_engine* e, int nsteps);
+smd_status smd_engine_sync(smd_engine* e);
+/* Current particle count of each replica (synchronises). */
+smd_status smd_engine_counts(smd_engine* e, unsigned* counts);
+unsigned smd_engine_num_replicas(const smd_engine* e);
+/* Sum over all steps so far of the particles entering each step (all
+ * replicas): the numerator of particle-steps/s. */
+smd_status smd_engine_particle_steps(smd_engine* e, unsigned long long* out);
+/* Copy replica r's particles to the host (at most cap). *n_out = count. */
+smd_status smd_engine_get(smd_engine* e, int r, float* pos4, int* types, unsigned cap, unsigned* n_out);
+/* Species histogram of replica r (counts[t] for t < num_types). */
+smd_status smd_engine_species_counts(smd_engine* e, int r, unsigned* counts);
+/* RNG stream state of replica r: 20 words per stream, stream-major
+ * (19 state words then iState). */
+smd_status smd_engine_get_streams(smd_engine* e, int r, uint32_t* words, unsigned cap_streams);
+smd_status smd_engine_set_streams(smd_engine* e, int r, const uint32_t* words, unsigned n_streams);
+/* Derived reaction parameters of replica r. */
+smd_status smd_engine_rxn(smd_engine* e, int r, smd_rxn_derived* out);
+/* Stage-level hooks used by the parity tests: run only the producer (or the
+ * producer + sort + cell table + candidate search) of one step and copy the
+ * intermediate arrays out.  stage: 1 = producer, 2 = + sort, 3 = + candidates,
+ * 4 = + resolve.  Arrays are sized for the current total particle count. */
+smd_status smd_engine_debug_stage(smd_engine* e, int stage, float* pos4, int* types, uint32_t* keys_sorted,
+                                  uint32_t* vals_sorted, int* cand, uint32_t* n_total, uint32_t* n_live);
+/* Number of buckets (keys deadKey = total buckets) and key bits. */
+smd_status smd_engine_layout(smd_engine* e, uint32_t* total_buckets, uint32_t* key_bits, uint32_t* total_streams,
+                             uint32_t* capacity);
+/* Per-kernel timing of the last timed step (HIP events, microseconds):
+ * names is a '\n' separated list written into buf. */
+smd_status smd_engine_profile_step(smd_engine* e, int nsteps, double* us, int max_stages, char* names, int names_cap,
+                                   int* n_stages);
+
+/* Draw `count` numbers from each of `n_streams` fresh streams initialised
+ * with (params record k % 32768, seeds[k]) on the GPU: out[k * count + j]. */
+smd_status smd_mt_generate(int device, const uint32_t* seeds, unsigned n_streams, int count, int batch3,
+                           uint32_t* out);
+
+/* Stable sort of (key, value) pairs on the GPU by the low key_bits bits
+ * (RadixSort::sort, radixsort.h:168-185).  Host arrays, n elements. */
+smd_status smd_sort_pairs(int device, uint32_t* keys, uint32_t* values, unsigned n, unsigned key_bits,
+                          int radix_bits);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPUSMOLDYN_H_INCLUDED */

@@ -1,0 +1,274 @@
+"""GPU parity: the HIP path through the C-ABI against the CPU oracle and the
+golden vectors.  Integer, index and RNG results must match bit for bit; so
+must positions, because both sides evaluate the same FP32 operations with
+explicit fused multiply-adds (the tolerance for positions is therefore 0)."""
+import os
+
+import numpy as np
+import pytest
+
+from gpusmoldyn_amd import BC_ABSORB, BC_CLAMP, BC_PERIODIC, BC_REFLECT, configs
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def banner(msg):
+    print(f"\n[gpu-test] {msg}", flush=True)
+
+
+def test_mt_matches_reference_dcmt(smd, gpu):
+    f = np.load(os.path.join(GOLD, "mt_streams.npz"))
+    streams, seeds, outs = f["streams"], f["seeds"], f["outputs"]
+    # stream k of smd_mt_generate uses record k % 32768: build 32768 streams
+    # whose seeds put the fixture seeds at the fixture stream ids.
+    S = 32768
+    all_seeds = np.arange(S, dtype=np.uint32) * np.uint32(2654435761)
+    all_seeds[streams] = seeds
+    for batch3 in (False, True):
+        banner(f"mt_generate S={S} count=512 batch3={batch3}")
+        got = smd.mt_generate(all_seeds, 512, device=gpu, batch3=batch3)
+        assert np.array_equal(got[streams], outs[:, :512])
+    banner("mt_generate 7 streams, 4096 draws")
+    got = smd.mt_generate(seeds, 4096, device=gpu)
+    # streams 0..6 use records 0..6: the fixture rows of records 0, 1, 2 line up
+    for row in range(3):
+        assert np.array_equal(got[row], outs[row])
+
+
+@pytest.mark.parametrize("radix_bits", [4, 8])
+@pytest.mark.parametrize("n,bits", [(0, 8), (1, 8), (4095, 12), (4096, 12), (4097, 20), (100003, 24), (1 << 20, 21),
+                                    (3000000, 25)])
+def test_sort_pairs_stable(smd, gpu, n, bits, radix_bits):
+    banner(f"sort n={n} bits={bits} radix={radix_bits}")
+    rng = np.random.default_rng(n + bits)
+    keys = rng.integers(0, 1 << bits, size=n, dtype=np.uint64).astype(np.uint32)
+    if n > 10:
+        keys[: n // 3] = keys[0]  # long runs of equal keys
+    vals = np.arange(n, dtype=np.uint32)
+    k, v = smd.sort_pairs(keys, vals, bits, radix_bits, device=gpu)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(k, keys[order])
+    assert np.array_equal(v, vals[order])
+
+
+# ---------------------------------------------------------------- workloads
+def w_c1_small():
+    spec, pos, types = configs.c1(nA=3000, nB=3000, kf=5.0, n_streams=2048, grid_bits=(5, 5, 5))
+    return spec, pos, types
+
+
+def w_c1_ref():
+    # the CPU-runnable workload itself, reference stream mapping (S = 32768)
+    return configs.c1()
+
+
+def w_periodic_reversible():
+    spec = configs.make_spec(
+        D=[0.0, 1.0, 1.0, 0.5], dt=1e-3,
+        reactions=[dict(order=2, reactants=[1, 2], products=[3], rate=20.0),
+                   dict(order=1, reactants=[3], products=[1, 2], rate=50.0, pgem=0.2),
+                   dict(order=1, reactants=[3], products=[], rate=5.0)],
+        box_lo=[0.0] * 3, box_hi=[5.0] * 3, faces=[BC_PERIODIC] * 6, grid_bits=(4, 4, 4),
+        num_streams=1000, unbind_offsets=1)
+    rng = np.random.default_rng(11)
+    n = 8000
+    pos = configs.uniform_box(rng, n, 0.0, 5.0)
+    types = rng.integers(1, 4, size=n).astype(np.int32)
+    return spec, pos, types
+
+
+def w_offsets():
+    # products placed at rotated unbinding offsets (reversible pair with a
+    # reachable geminate probability): exercises sincos/theta bit parity
+    spec = configs.make_spec(
+        D=[0.0, 0.01, 0.01, 0.005], dt=1e-1,
+        reactions=[dict(order=2, reactants=[1, 2], products=[3], rate=0.05),
+                   dict(order=1, reactants=[3], products=[1, 2], rate=3.0, pgem=0.2)],
+        box_lo=[0.0] * 3, box_hi=[4.0] * 3, faces=[BC_REFLECT] * 6, grid_bits=(4, 4, 4),
+        num_streams=700, unbind_offsets=1)
+    rng = np.random.default_rng(12)
+    n = 6000
+    pos = configs.uniform_box(rng, n, 0.0, 4.0)
+    types = rng.integers(1, 4, size=n).astype(np.int32)
+    return spec, pos, types
+
+
+def w_c3_small():
+    spec, pos, types = configs.c3(n=60000, n_blobs=8, sigma=1.0, n_streams=4096, grid_bits=(5, 5, 5), cell=0.15,
+                                  L=20.0)
+    spec.sphere[0][3] = 6.0
+    spec.rxn[0].rate = 2.0
+    return spec, pos, types
+
+
+def w_clamp():
+    spec = configs.make_spec(
+        D=[0.0, 0.3, 0.1], dt=1e-2, reactions=[], box_lo=[-1.0] * 3, box_hi=[1.0] * 3, faces=[BC_CLAMP] * 6,
+        grid_bits=(6, 6, 6), particle_radius=1.0 / 64.0)
+    rng = np.random.default_rng(13)
+    n = 20000
+    pos = configs.uniform_box(rng, n, -1.0, 1.0)
+    types = np.where(np.arange(n) < 3 * n // 4, 1, 2).astype(np.int32)
+    return spec, pos, types
+
+
+WORKLOADS = {
+    "c1_small": w_c1_small,
+    "periodic_reversible": w_periodic_reversible,
+    "offsets": w_offsets,
+    "c3_small": w_c3_small,
+    "clamp_reference_test3": w_clamp,
+}
+
+
+def _engine(smd, spec, pos, types, radix_bits=8):
+    e = smd.Engine(0, radix_bits)
+    e.add_replica(spec, pos, types)
+    e.finalize()
+    return e
+
+
+@pytest.mark.parametrize("name", list(WORKLOADS))
+def test_stage_parity(smd, orc, gpu, name):
+    spec, pos, types = WORKLOADS[name]()
+    g = smd.gauss_table()
+    n = len(types)
+    has_bim = any(spec.rxn[k].order == 2 for k in range(spec.num_reactions))
+    for stage in ((1, 2, 3, 4) if has_bim else (1,)):
+        banner(f"stage {stage} workload {name} n={n}")
+        e = _engine(smd, spec, pos, types)
+        got = e.debug_stage(stage, n)
+        e.close()
+        sim = orc.sim_for(smd, spec, pos, types)
+        ref = sim.stage(g, stage)
+        assert np.array_equal(got["type"], ref["type"]), "types"
+        assert np.array_equal(got["pos"].view(np.uint32), ref["pos"].view(np.uint32)), "positions"
+        assert np.array_equal(got["keys"], ref["keys"]), "keys"
+        if stage >= 2:
+            assert got["n_live"] == ref["n_live"]
+            assert np.array_equal(got["vals"], ref["vals"]), "sorted index"
+        if stage >= 3:
+            assert np.array_equal(got["cand"], ref["cand"]), "pair candidates"
+
+
+@pytest.mark.parametrize("name,steps", [("c1_small", 60), ("periodic_reversible", 40), ("offsets", 40),
+                                        ("c3_small", 20), ("clamp_reference_test3", 30)])
+def test_multistep_parity(smd, orc, gpu, name, steps):
+    spec, pos, types = WORKLOADS[name]()
+    banner(f"multistep {name} steps={steps}")
+    g = smd.gauss_table()
+    e = _engine(smd, spec, pos, types)
+    sim = orc.sim_for(smd, spec, pos, types, cap=8 * len(types) + 1024)
+    for k in range(steps):
+        e.step(1)
+        sim.step(g, 1)
+        if k % 10 == 9 or k == steps - 1:
+            gp, gt = e.get(0)
+            rp, rt = sim.state()
+            assert len(gt) == len(rt), f"count at step {k}"
+            assert np.array_equal(gt, rt), f"types at step {k}"
+            assert np.array_equal(gp.view(np.uint32), rp.view(np.uint32)), f"positions at step {k}"
+    assert np.array_equal(e.get_streams(0), sim.mt), "stream states"
+    e.close()
+
+
+def test_c1_full_run_matches_oracle(smd, orc, gpu):
+    """The CPU-runnable workload end to end (1000 steps, S = 32768 so that
+    particle i uses stream i exactly as the reference does for N <= 32768)."""
+    spec, pos, types = w_c1_ref()
+    banner("c1 full 1000 steps")
+    g = smd.gauss_table()
+    e = _engine(smd, spec, pos, types)
+    e.step(configs.C1_STEPS)
+    sim = orc.sim_for(smd, spec, pos, types)
+    sim.step(g, configs.C1_STEPS)
+    gp, gt = e.get(0)
+    rp, rt = sim.state()
+    assert np.array_equal(gt, rt)
+    assert np.array_equal(gp.view(np.uint32), rp.view(np.uint32))
+    # A + B -> C happened (k nA nB / V dt ~ 8.4/s)
+    assert (gt == 3).sum() > 0
+
+
+def test_ensemble_equals_alone(smd, gpu):
+    reps = [w_c1_small(), w_periodic_reversible(), w_offsets(), w_clamp()]
+    banner("ensemble of 4 vs alone")
+    e = smd.Engine(0)
+    for spec, pos, types in reps:
+        e.add_replica(spec, pos, types)
+    e.finalize()
+    e.step(25)
+    for r, (spec, pos, types) in enumerate(reps):
+        a = _engine(smd, spec, pos, types)
+        a.step(25)
+        ap, at = a.get(0)
+        bp, bt = e.get(r)
+        assert np.array_equal(at, bt), f"replica {r} types"
+        assert np.array_equal(ap.view(np.uint32), bp.view(np.uint32)), f"replica {r} positions"
+        a.close()
+    e.close()
+
+
+def test_radix4_equals_radix8(smd, gpu):
+    spec, pos, types = w_periodic_reversible()
+    banner("radix 4 vs 8")
+    a = _engine(smd, spec, pos, types, radix_bits=4)
+    b = _engine(smd, spec, pos, types, radix_bits=8)
+    a.step(15)
+    b.step(15)
+    ap, at = a.get(0)
+    bp, bt = b.get(0)
+    assert np.array_equal(at, bt) and np.array_equal(ap.view(np.uint32), bp.view(np.uint32))
+
+
+def test_checkpoint_resume(smd, gpu):
+    spec, pos, types = w_periodic_reversible()
+    banner("checkpoint/resume")
+    a = _engine(smd, spec, pos, types)
+    a.step(10)
+    p10, t10 = a.get(0)
+    st10 = a.get_streams(0)
+    a.step(10)
+    p20, t20 = a.get(0)
+    b = _engine(smd, spec, p10, t10)
+    b.set_streams(0, st10)
+    b.step(10)
+    q20, u20 = b.get(0)
+    assert np.array_equal(t20, u20) and np.array_equal(p20.view(np.uint32), q20.view(np.uint32))
+
+
+def test_growth_beyond_capacity(smd, orc, gpu):
+    # A -> 2A doubles the population every few steps: the engine must grow its
+    # buffers without losing or reordering particles
+    spec = configs.make_spec(D=[0.0, 0.2], dt=1e-2, reactions=[dict(order=1, reactants=[1], products=[1, 1],
+                                                                     rate=40.0)],
+                             box_lo=[0.0] * 3, box_hi=[10.0] * 3, faces=[BC_REFLECT] * 6, grid_bits=(3, 3, 3),
+                             num_streams=256)
+    pos = configs.uniform_box(np.random.default_rng(21), 500, 0.0, 10.0)
+    types = np.ones(500, np.int32)
+    banner("growth")
+    e = smd.Engine(0)
+    e.add_replica(spec, pos, types)
+    e.finalize(capacity=1024)
+    sim = orc.sim_for(smd, spec, pos, types, cap=1 << 20)
+    g = smd.gauss_table()
+    e.step(12)
+    sim.step(g, 12)
+    gp, gt = e.get(0)
+    rp, rt = sim.state()
+    assert len(gt) > 1024
+    assert np.array_equal(gt, rt) and np.array_equal(gp.view(np.uint32), rp.view(np.uint32))
+
+
+def test_extinction_and_empty(smd, gpu):
+    spec = configs.make_spec(D=[0.0, 0.2], dt=1e-1, reactions=[dict(order=1, reactants=[1], products=[], rate=100.0)],
+                             box_lo=[0.0] * 3, box_hi=[10.0] * 3, faces=[BC_REFLECT] * 6, grid_bits=(3, 3, 3),
+                             num_streams=64)
+    pos = configs.uniform_box(np.random.default_rng(22), 300, 0.0, 10.0)
+    banner("extinction")
+    e = _engine(smd, spec, pos, np.ones(300, np.int32))
+    e.step(5)
+    assert e.counts()[0] == 0
+    e.step(3)  # stepping an empty system is a no-op
+    assert e.counts()[0] == 0

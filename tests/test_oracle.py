@@ -1,0 +1,203 @@
+"""Pins of the CPU oracle itself: math kernels against libm, the cell-list pair
+search against O(N^2) brute force, the sort against a stable argsort, and the
+step's statistics against closed forms (diffusion moments, first-order decay
+and branching, bimolecular mass action)."""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+from gpusmoldyn_amd import BC_ABSORB, BC_CLAMP, BC_PERIODIC, BC_REFLECT, configs
+
+
+def test_sincos_and_theta_against_libm(orc):
+    L = orc.lib()
+    xs = np.linspace(0.0, 2 * math.pi, 20001, dtype=np.float32)
+    s, c = ctypes.c_float(), ctypes.c_float()
+    err = 0.0
+    for x in xs:
+        L.orc_sincos(float(x), ctypes.byref(s), ctypes.byref(c))
+        err = max(err, abs(s.value - math.sin(float(x))), abs(c.value - math.cos(float(x))))
+    assert err < 4e-7
+    us = np.concatenate([np.linspace(0, 1, 20001), [1e-9, 1e-6, 0.25, 0.2500001, 0.9999999]]).astype(np.float32)
+    errt = max(abs(L.orc_theta(float(u)) - math.acos(1.0 - 2.0 * float(u))) for u in us)
+    assert errt < 2e-6
+
+
+def _small_bim_spec(periodic, n_streams=128, L=4.0, k=2.0, grid=(3, 3, 3)):
+    faces = [BC_PERIODIC] * 6 if periodic else [BC_REFLECT] * 6
+    return configs.make_spec(
+        D=[0.0, 1.0, 1.0, 0.5], dt=1e-3,
+        reactions=[dict(order=2, reactants=[1, 2], products=[3], rate=k)],
+        box_lo=[0.0] * 3, box_hi=[L] * 3, faces=faces, grid_bits=grid, num_streams=n_streams)
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_cell_search_equals_bruteforce(orc, smd, periodic):
+    spec = _small_bim_spec(periodic)
+    rng = np.random.default_rng(1)
+    n = 3000
+    pos = configs.uniform_box(rng, n, 0.0, 4.0)
+    types = rng.integers(1, 3, size=n).astype(np.int32)
+    sim = orc.sim_for(smd, spec, pos, types)
+    st = sim.stage(smd.gauss_table(), 3)
+    assert st["n_live"] == n
+    spt = np.zeros((n, 4), np.float32)
+    spt[:, :3] = st["pos"][st["vals"], :3]
+    spt[:, 3] = st["type"][st["vals"]].view(np.float32)
+    bf = np.zeros(n, np.int32)
+    orc.lib().orc_candidates_bruteforce(ctypes.byref(sim.s.net), spt.ctypes.data_as(ctypes.c_void_p), n,
+                                        bf.ctypes.data_as(ctypes.c_void_p))
+    assert (bf >= 0).sum() > 50  # the test is not vacuous
+    assert np.array_equal(st["cand"], bf)
+
+
+def test_sort_is_stable_by_bucket(orc, smd):
+    spec = _small_bim_spec(False)
+    rng = np.random.default_rng(2)
+    n = 5000
+    pos = configs.uniform_box(rng, n, 0.0, 4.0)
+    types = rng.integers(1, 3, size=n).astype(np.int32)
+    sim = orc.sim_for(smd, spec, pos, types)
+    k1 = sim.stage(smd.gauss_table(), 1)["keys"]
+    sim2 = orc.sim_for(smd, spec, pos, types)
+    st = sim2.stage(smd.gauss_table(), 2)
+    order = np.argsort(k1, kind="stable")
+    assert np.array_equal(st["vals"], order)
+    assert np.array_equal(st["keys"], k1[order])
+
+
+def _diffusion_spec(D=1.0, dt=1e-3, n_streams=0, faces=None, L=1000.0):
+    return configs.make_spec(D=[0.0, D], dt=dt, reactions=[], box_lo=[0.0] * 3, box_hi=[L] * 3,
+                             faces=faces or [BC_REFLECT] * 6, grid_bits=(4, 4, 4), num_streams=n_streams)
+
+
+def test_diffusion_moments(orc, smd):
+    spec = _diffusion_spec(D=1.0, dt=1e-3)
+    n = 200000
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = 500.0
+    pos[:, 3] = 1.0
+    sim = orc.sim_for(smd, spec, pos, np.ones(n, np.int32))
+    g = smd.gauss_table()
+    sim.step(g, 1)
+    p, _ = sim.state()
+    d = p[:, :3].astype(np.float64) - 500.0
+    rate = float(np.sqrt(np.float32(2.0) * np.float32(1.0) * np.float32(1e-3)))
+    var = rate * rate * 0.9996794
+    # every displacement is rate * a table entry (500 + x is exact to ~3e-5)
+    assert abs(d.mean()) < 5 * math.sqrt(var / (3 * n))
+    assert abs(d.var() / var - 1.0) < 5 * math.sqrt(2.0 / (3 * n))
+    assert np.all(p[:, 3] == 1.0)
+
+
+def test_first_order_decay_and_branching(orc, smd):
+    k1, k2, dt = 30.0, 10.0, 1e-2
+    spec = configs.make_spec(
+        D=[0.0, 0.0, 0.0, 0.0], dt=dt,
+        reactions=[dict(order=1, reactants=[1], products=[2], rate=k1),
+                   dict(order=1, reactants=[1], products=[], rate=k2)],
+        box_lo=[0.0] * 3, box_hi=[10.0] * 3, faces=[BC_REFLECT] * 6, grid_bits=(2, 2, 2), num_streams=4096)
+    der = smd.derive(spec)
+    p_tot = 1.0 - math.exp(-(k1 + k2) * dt)
+    p1 = k1 / (k1 + k2) * p_tot
+    assert abs(der[0].prob - p1) < 1e-6
+    assert abs(der[1].prob - (k2 / (k1 + k2) * p_tot) / (1.0 - p1)) < 1e-6
+    n = 100000
+    pos = configs.uniform_box(np.random.default_rng(3), n, 1.0, 9.0)
+    sim = orc.sim_for(smd, spec, pos, np.ones(n, np.int32))
+    sim.step(smd.gauss_table(), 1)
+    _, t = sim.state()
+    nB = (t == 2).sum()
+    nA = (t == 1).sum()
+    ndead = n - len(t)
+    for cnt, p in ((nB, p1), (ndead, k2 / (k1 + k2) * p_tot), (nA, 1 - p_tot)):
+        assert abs(cnt - n * p) < 5 * math.sqrt(n * p * (1 - p))
+
+
+def test_split_appends_births_in_parent_order(orc, smd):
+    spec = configs.make_spec(
+        D=[0.0, 0.1, 0.1, 0.1], dt=1e-2,
+        reactions=[dict(order=1, reactants=[1], products=[2, 3], rate=20.0)],
+        box_lo=[0.0] * 3, box_hi=[10.0] * 3, faces=[BC_REFLECT] * 6, grid_bits=(2, 2, 2), num_streams=512)
+    n = 4000
+    pos = configs.uniform_box(np.random.default_rng(4), n, 1.0, 9.0)
+    sim = orc.sim_for(smd, spec, pos, np.ones(n, np.int32))
+    sim.step(smd.gauss_table(), 1)
+    p, t = sim.state()
+    nsplit = (t[:n] == 2).sum()
+    assert len(t) == n + nsplit
+    assert np.all(t[n:] == 3)
+    # each birth sits at its parent's pre-boundary position (same point)
+    parents = np.nonzero(t[:n] == 2)[0]
+    assert np.allclose(p[n:, :3], p[parents, :3], atol=1e-5)
+
+
+def test_bimolecular_mass_action(orc, smd):
+    # well-mixed regime (rms step / binding radius ~ 10): the simulated
+    # reaction count per step follows k nA nB / V dt
+    k, dt, L = 2e-2, 1e-3, 10.0
+    spec = configs.make_spec(
+        D=[0.0, 1.0, 1.0, 0.5], dt=dt,
+        reactions=[dict(order=2, reactants=[1, 2], products=[3], rate=k)],
+        box_lo=[0.0] * 3, box_hi=[L] * 3, faces=[BC_PERIODIC] * 6, grid_bits=(5, 5, 5), num_streams=32768)
+    n = 40000
+    rng = np.random.default_rng(5)
+    pos = configs.uniform_box(rng, n, 0.0, L)
+    types = np.array([1, 2] * (n // 2), np.int32)
+    sim = orc.sim_for(smd, spec, pos, types)
+    g = smd.gauss_table()
+    nsteps = 20
+    expected = 0.0
+    nA = nB = n // 2
+    for _ in range(nsteps):
+        expected += k * nA * nB / L ** 3 * dt
+        sim.step(g, 1)
+        _, t = sim.state()
+        nA, nB = (t == 1).sum(), (t == 2).sum()
+    got = n // 2 - nA
+    assert abs(got - expected) < 4 * math.sqrt(expected) + 0.03 * expected
+
+
+def test_boundaries(orc, smd):
+    # reflective faces keep particles inside; absorbing -z removes; periodic wraps
+    for faces, check in (
+        ([BC_REFLECT] * 6, "inside"),
+        ([BC_PERIODIC] * 6, "inside"),
+        ([BC_REFLECT] * 4 + [BC_ABSORB, BC_REFLECT], "absorb"),
+        ([BC_CLAMP] * 6, "clamp"),
+    ):
+        spec = _diffusion_spec(D=5.0, dt=1e-2, faces=faces, L=2.0, n_streams=1024)  # max step 1.16 < box
+        spec.particle_radius = 1.0 / 64.0
+        n = 20000
+        pos = configs.uniform_box(np.random.default_rng(6), n, 0.0, 2.0)
+        sim = orc.sim_for(smd, spec, pos, np.ones(n, np.int32))
+        sim.step(smd.gauss_table(), 3)
+        p, t = sim.state()
+        if check == "inside":
+            assert len(t) == n and p[:, :3].min() >= 0.0 and p[:, :3].max() < 2.0
+        elif check == "absorb":
+            assert len(t) < n and p[:, 2].min() >= 0.0 and p[:, :3].max() <= 2.0
+        else:
+            r = np.float32(1.0 / 64.0)
+            assert p[:, :3].min() >= np.float32(0.0) + r and p[:, :3].max() <= np.float32(2.0) - r
+
+
+def test_sphere_reflects_both_ways(orc, smd):
+    spec = _diffusion_spec(D=20.0, dt=1e-2, L=10.0, n_streams=2048)
+    spec.num_spheres = 1
+    for q, v in enumerate((5.0, 5.0, 5.0, 2.0)):
+        spec.sphere[0][q] = v
+    rng = np.random.default_rng(7)
+    n = 20000
+    pos = configs.uniform_box(rng, n, 0.5, 9.5)
+    r0 = np.linalg.norm(pos[:, :3] - 5.0, axis=1)
+    keep = np.abs(r0 - 2.0) > 1e-3
+    pos = pos[keep]
+    inside0 = r0[keep] < 2.0
+    sim = orc.sim_for(smd, spec, pos, np.ones(len(pos), np.int32))
+    sim.step(smd.gauss_table(), 5)
+    p, _ = sim.state()
+    inside = np.linalg.norm(p[:, :3].astype(np.float64) - 5.0, axis=1) < 2.0
+    assert np.array_equal(inside, inside0)
