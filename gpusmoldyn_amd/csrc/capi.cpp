@@ -29,10 +29,6 @@ std::string default_data_dir() {
 using smd::Engine;
 using smd::set_error;
 
-struct smd_engine {
-  Engine* impl;
-};
-
 #define CAPI_HIPCHK(x)                                                              \
   do {                                                                              \
     hipError_t e_ = (x);                                                            \

@@ -480,6 +480,36 @@ smd_status Engine::sync() {
   return SMD_OK;
 }
 
+smd_status Engine::set_particles(const float* pos4, const int* types, uint32_t n) {
+  if (!finalized || R != 1) {
+    set_error("set_particles needs a finalized single-replica engine");
+    return SMD_ERR_INVALID;
+  }
+  for (uint32_t i = 0; i < n; ++i)
+    if (types[i] < 1 || types[i] >= specs[0].num_types) {
+      set_error("particle type out of range");
+      return SMD_ERR_INVALID;
+    }
+  smd_status st = sync();
+  if (st != SMD_OK) return st;
+  if ((uint64_t)n * (anySplit ? 2 : 1) > cap) {
+    knownN = 0;
+    st = grow(4ull * n);
+    if (st != SMD_OK) return st;
+  }
+  if (n) {
+    HIPCHK(hipMemcpy(dPos[cur], pos4, n * sizeof(float4), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dType[cur], types, n * sizeof(int), hipMemcpyHostToDevice));
+  }
+  const uint32_t off[2] = {0u, n};
+  HIPCHK(hipMemcpy(dRepOff, off, 2 * sizeof(uint32_t), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dRepCount, &n, sizeof(uint32_t), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dN, &n, sizeof(uint32_t), hipMemcpyHostToDevice));
+  Nub = n;
+  knownN = n;
+  return SMD_OK;
+}
+
 smd_status Engine::particle_steps(unsigned long long* out) {
   smd_status st = sync();
   if (st != SMD_OK) return st;

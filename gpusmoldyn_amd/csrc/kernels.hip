@@ -207,7 +207,7 @@ __global__ __launch_bounds__(256) void k_sort_up(const uint32_t* __restrict__ ke
                                                  uint32_t* __restrict__ counts, uint32_t stride, int shift) {
   constexpr int RADIX = 1 << BITS;
   __shared__ uint32_t hist[RADIX];
-  const uint32_t n = *dN;
+  const uint32_t n = min(*dN, (stride - 1u) * (uint32_t)kSortTile);  // never beyond the sized grid
   const uint32_t tile = blockIdx.x;
   const uint32_t nTiles = (n + kSortTile - 1) / kSortTile;
   if (tile >= nTiles) return;
@@ -247,7 +247,7 @@ __global__ __launch_bounds__(256) void k_sort_rowscan(uint32_t* __restrict__ cou
                                                       const uint32_t* __restrict__ dN, uint32_t stride) {
   __shared__ uint32_t wsum[4];
   __shared__ uint32_t tot;
-  const uint32_t n = *dN;
+  const uint32_t n = min(*dN, (stride - 1u) * (uint32_t)kSortTile);
   const uint32_t nTiles = (n + kSortTile - 1) / kSortTile;
   const uint32_t d = blockIdx.x;
   uint32_t* row = counts + (size_t)d * stride;
@@ -281,7 +281,7 @@ __global__ __launch_bounds__(256) void k_sort_down(const uint32_t* __restrict__ 
   __shared__ uint32_t wsum[4];
   __shared__ uint32_t tot;
 
-  const uint32_t n = *dN;
+  const uint32_t n = min(*dN, (stride - 1u) * (uint32_t)kSortTile);
   const uint32_t tile = blockIdx.x;
   const uint32_t nTiles = (n + kSortTile - 1) / kSortTile;
   if (tile >= nTiles) return;
@@ -395,22 +395,45 @@ __device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* __restrict__
   return lo;
 }
 
+// A workgroup covers kFirstSpan consecutive buckets: two threads locate the
+// span's key range [lo, hi) by binary search, the range is staged in LDS when
+// it fits, and every bucket's lower bound is then a short search in LDS.
+constexpr uint32_t kFirstSpan = 4096;
+constexpr uint32_t kFirstStage = 8192;
+
 __global__ __launch_bounds__(256) void k_first(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ dN,
                                                uint32_t nb, uint32_t* __restrict__ first,
                                                uint32_t* __restrict__ dLive) {
-  constexpr uint32_t kPer = 16;
+  __shared__ uint32_t s_keys[kFirstStage];
+  __shared__ uint32_t s_lo, s_hi;
   const uint32_t n = *dN;
-  const uint32_t h0 = (blockIdx.x * 256 + threadIdx.x) * kPer;
-  if (h0 >= nb) return;
-  const uint32_t h1 = min(h0 + kPer, nb);
-  const uint32_t lo = lower_bound_u32(keys, 0, n, h0);
-  const uint32_t hi = lower_bound_u32(keys, lo, n, h1);
-  for (uint32_t h = h0; h < h1; ++h) {
-    const uint32_t f = lower_bound_u32(keys, lo, hi, h);
+  const uint32_t h0 = blockIdx.x * kFirstSpan;
+  const uint32_t h1 = min(h0 + kFirstSpan, nb);
+  if (threadIdx.x == 0) s_lo = lower_bound_u32(keys, 0, n, h0);
+  if (threadIdx.x == 64) s_hi = lower_bound_u32(keys, 0, n, h1);
+  __syncthreads();
+  const uint32_t lo = s_lo, hi = s_hi, m = hi - lo;
+  const bool staged = m <= kFirstStage;
+  if (staged)
+    for (uint32_t k = threadIdx.x; k < m; k += 256) s_keys[k] = keys[lo + k];
+  __syncthreads();
+  for (uint32_t h = h0 + threadIdx.x; h < h1; h += 256) {
+    uint32_t f;
+    if (staged) {
+      uint32_t a = 0, b = m;
+      while (a < b) {
+        const uint32_t mid = (a + b) >> 1;
+        if (s_keys[mid] < h) a = mid + 1;
+        else b = mid;
+      }
+      f = lo + a;
+    } else {
+      f = lower_bound_u32(keys, lo, hi, h);
+    }
     first[h] = f;
     if (h == nb - 1) *dLive = f;
   }
-  if (h1 == nb) first[nb] = n;
+  if (h1 == nb && threadIdx.x == 0) first[nb] = n;
 }
 
 // sortedPT[j] = (pos[val[j]].xyz, type[val[j]] as float bits) for live j.
@@ -442,61 +465,184 @@ __device__ __forceinline__ int replica_of_bucket(const uint32_t* __restrict__ bu
   return lo;
 }
 
-__device__ __forceinline__ void scan_range(const ReplicaTables& T, const float4* __restrict__ spt, uint32_t a,
-                                           uint32_t b, uint32_t j, float px, float py, float pz, int tj,
-                                           int& best, float& bestD2) {
-  for (uint32_t k = a; k < b; ++k) {
-    if (k == j) continue;
-    const float4 q = spt[k];
-    const int tk = __float_as_int(q.w);
-    if (tk <= 0 || tk >= T.numTypes) continue;
-    const int rx = T.pairRxn[tj * kMaxTypes + tk];
-    if (rx < 0) continue;
-    const float dx = min_image(px - q.x, T, 0);
-    const float dy = min_image(py - q.y, T, 1);
-    const float dz = min_image(pz - q.z, T, 2);
-    const float d2 = dot3(dx, dy, dz, dx, dy, dz);
-    if (d2 <= T.bindR2[rx] && (d2 < bestD2 || (d2 == bestD2 && (int)k < best))) {
-      best = (int)k;
-      bestD2 = d2;
-    }
+// Candidate test: a cheap type-mask and largest-radius filter first, the
+// reaction table only for pairs already within the largest binding radius.
+__device__ __forceinline__ void consider(const ReplicaTables& T, const float4 q, uint32_t k, uint32_t j, float px,
+                                         float py, float pz, int tj, uint32_t mask, float maxB2, int& best,
+                                         float& bestD2) {
+  const int tk = __float_as_int(q.w);
+  if (k == j || tk <= 0 || tk >= kMaxTypes || !((mask >> tk) & 1u)) return;
+  const float dx = min_image(px - q.x, T, 0);
+  const float dy = min_image(py - q.y, T, 1);
+  const float dz = min_image(pz - q.z, T, 2);
+  const float d2 = dot3(dx, dy, dz, dx, dy, dz);
+  if (d2 > maxB2) return;
+  const int rx = T.pairRxn[tj * kMaxTypes + tk];
+  if (d2 <= T.bindR2[rx] && (d2 < bestD2 || (d2 == bestD2 && (int)k < best))) {
+    best = (int)k;
+    bestD2 = d2;
   }
 }
 
+__device__ __forceinline__ void scan_range(const ReplicaTables& T, const float4* __restrict__ spt, uint32_t a,
+                                           uint32_t b, uint32_t j, float px, float py, float pz, int tj,
+                                           uint32_t mask, float maxB2, int& best, float& bestD2) {
+  uint32_t k = a;
+  for (; k + 2 <= b; k += 2) {  // two loads in flight
+    const float4 q0 = spt[k];
+    const float4 q1 = spt[k + 1];
+    consider(T, q0, k, j, px, py, pz, tj, mask, maxB2, best, bestD2);
+    consider(T, q1, k + 1, j, px, py, pz, tj, mask, maxB2, best, bestD2);
+  }
+  if (k < b) consider(T, spt[k], k, j, px, py, pz, tj, mask, maxB2, best, bestD2);
+}
+
+// One run [a, b) of sorted indices, read from the LDS chunk [cs, ce) when it
+// is staged there, else from global memory.
+__device__ __forceinline__ void scan_run(const ReplicaTables& T, const float4* __restrict__ spt,
+                                         const float4* chunk, uint32_t cs, bool staged, uint32_t a, uint32_t b,
+                                         uint32_t j, float px, float py, float pz, int tj, uint32_t mask,
+                                         float maxB2, int& best, float& bestD2) {
+  if (staged) {
+    for (uint32_t k = a; k < b; ++k)
+      consider(T, chunk[k - cs], k, j, px, py, pz, tj, mask, maxB2, best, bestD2);
+  } else {
+    scan_range(T, spt, a, b, j, px, py, pz, tj, mask, maxB2, best, bestD2);
+  }
+}
+
+// LDS-tiled 27-neighbour search.  A workgroup takes 256 consecutive sorted
+// particles of one replica (keys are non-decreasing in j).  For a (dz, dy)
+// neighbour-row offset that wraps for none of them, every particle's row is
+// its own row shifted by a constant, so the buckets needed by the whole
+// workgroup lie in the whole rows from the first particle's shifted row to the
+// last one's: one contiguous range of sorted particles.  Those ranges are
+// staged in LDS three at a time and each particle scans its bucket runs from
+// LDS.  Offsets that wrap at a grid edge, ranges longer than a chunk, and
+// particles of a second replica in the workgroup read global memory instead.
+constexpr uint32_t kCandChunk = 768;
+
+__device__ __forceinline__ uint32_t row_of(uint32_t key, const ReplicaTables& T, int q, bool& wraps) {
+  const uint32_t bx = T.gbits[0], by = T.gbits[1];
+  const uint32_t my = (1u << by) - 1u, mz = (1u << T.gbits[2]) - 1u;
+  const uint32_t local = key - T.bucketOff;
+  const int gy = (int)((local >> bx) & my) + q % 3 - 1;
+  const int gz = (int)(local >> (bx + by)) + q / 3 - 1;
+  wraps = gy < 0 || gy > (int)my || gz < 0 || gz > (int)mz;
+  return T.bucketOff + (((uint32_t)gz & mz) << (bx + by)) + (((uint32_t)gy & my) << bx);
+}
+
 __global__ __launch_bounds__(256) void k_candidates(CandArgs a) {
-  const uint32_t j = blockIdx.x * 256 + threadIdx.x;
+  __shared__ float4 s_pt[3][kCandChunk];
+  __shared__ uint32_t s_cs[9];
+  __shared__ int s_len[9];  // staged length + 1, 0 = not staged
+  __shared__ uint32_t s_cnt[4], s_wrap[4];
   const uint32_t nLive = *a.dLive;
-  if (j >= nLive) return;
-  const uint32_t key = a.keys[j];
-  const int r = replica_of_bucket(a.bucketStart, a.R, key);
+  const uint32_t j0 = blockIdx.x * 256;
+  if (j0 >= nLive) return;
+  const uint32_t j = j0 + threadIdx.x;
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  const bool active = j < nLive;
+  const uint32_t key0 = a.keys[j0];
+  const int r0 = replica_of_bucket(a.bucketStart, a.R, key0);
+
+  uint32_t key = key0;
+  int r = r0;
+  float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (active) {
+    key = a.keys[j];
+    r = replica_of_bucket(a.bucketStart, a.R, key);
+    p = a.sortedPT[j];
+  }
   const ReplicaTables& T = a.tables[r];
+  const int tj = __float_as_int(p.w);
+  const uint32_t mask = (active && T.hasBim && tj > 0 && tj < kMaxTypes) ? T.partnerMask[tj] : 0u;
+  const float maxB2 = mask ? T.maxBindR2[tj] : 0.0f;
+  const uint32_t bx = T.gbits[0];
+  const uint32_t mx = (1u << bx) - 1u;
+  const uint32_t gx = (key - T.bucketOff) & mx;
+  const bool inner = gx >= 1 && gx < mx;
+  uint32_t rowBase[9];
+  uint32_t wrapBits = 0;
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    bool wr;
+    rowBase[q] = row_of(key, T, q, wr);
+    wrapBits |= (wr ? 1u : 0u) << q;
+  }
+  // Run bounds of the inner case, all requested before any scanning.
+  const uint32_t* __restrict__ first = a.first;
+  uint32_t lo[9], hi[9];
+  if (mask && inner) {
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      lo[q] = first[rowBase[q] + gx - 1u];
+      hi[q] = first[rowBase[q] + gx + 2u];
+    }
+  }
+  // Workgroup extent in replica r0 (a prefix of the workgroup) and the union
+  // of wrapping offsets.
+  const bool isR0 = active && r == r0;
+  const uint64_t bal = __ballot(isR0);
+  uint32_t wb = isR0 ? wrapBits : 0u;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) wb |= __shfl_xor(wb, o, 64);
+  if (lane == 0) {
+    s_cnt[w] = (uint32_t)__popcll(bal);
+    s_wrap[w] = wb;
+  }
+  __syncthreads();
+  if (threadIdx.x < 9) {
+    const int q = (int)threadIdx.x;
+    const uint32_t nR0 = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+    const uint32_t wrapAll = s_wrap[0] | s_wrap[1] | s_wrap[2] | s_wrap[3];
+    int len = 0;
+    uint32_t cs = 0;
+    if (nR0 > 0 && !((wrapAll >> q) & 1u)) {
+      const ReplicaTables& T0 = a.tables[r0];
+      bool wr;
+      const uint32_t b0 = row_of(key0, T0, q, wr);
+      const uint32_t b1 = row_of(a.keys[j0 + nR0 - 1u], T0, q, wr) + ((1u << T0.gbits[0]) - 1u);
+      cs = first[b0];
+      const uint32_t ce = first[b1 + 1u];
+      len = (ce - cs) <= kCandChunk ? (int)(ce - cs) + 1 : 0;
+    }
+    s_cs[q] = cs;
+    s_len[q] = len;
+  }
+  __syncthreads();
+
   int best = -1;
-  if (T.hasBim) {
-    const float4 p = a.sortedPT[j];
-    const int tj = __float_as_int(p.w);
-    float bestD2 = 3.4e38f;
-    const uint32_t local = key - T.bucketOff;
-    const uint32_t bx = T.gbits[0], by = T.gbits[1];
-    const uint32_t mx = (1u << bx) - 1u, my = (1u << by) - 1u, mz = (1u << T.gbits[2]) - 1u;
-    const uint32_t gx = local & mx, gy = (local >> bx) & my, gz = local >> (bx + by);
-    const uint32_t* __restrict__ first = a.first;
-    for (int dz = -1; dz <= 1; ++dz) {
-      const uint32_t zz = (gz + (uint32_t)dz) & mz;
-      for (int dy = -1; dy <= 1; ++dy) {
-        const uint32_t yy = (gy + (uint32_t)dy) & my;
-        const uint32_t row = T.bucketOff + (zz << (bx + by)) + (yy << bx);
-        if (gx >= 1 && gx < mx) {
-          scan_range(T, a.sortedPT, first[row + gx - 1], first[row + gx + 2], j, p.x, p.y, p.z, tj, best, bestD2);
+  float bestD2 = 3.4e38f;
+  for (int ph = 0; ph < 3; ++ph) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int len = s_len[3 * ph + c] - 1;
+      const uint32_t cs = s_cs[3 * ph + c];
+      for (int k = (int)threadIdx.x; k < len; k += 256) s_pt[c][k] = a.sortedPT[cs + (uint32_t)k];
+    }
+    __syncthreads();
+    if (mask) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const int q = 3 * ph + c;
+        const bool staged = isR0 && s_len[q] > 0;
+        const uint32_t cs = s_cs[q];
+        if (inner) {
+          scan_run(T, a.sortedPT, s_pt[c], cs, staged, lo[q], hi[q], j, p.x, p.y, p.z, tj, mask, maxB2, best,
+                   bestD2);
         } else {
           for (int dx = -1; dx <= 1; ++dx) {
             const uint32_t xx = (gx + (uint32_t)dx) & mx;
-            scan_range(T, a.sortedPT, first[row + xx], first[row + xx + 1], j, p.x, p.y, p.z, tj, best, bestD2);
+            scan_run(T, a.sortedPT, s_pt[c], cs, staged, first[rowBase[q] + xx], first[rowBase[q] + xx + 1u], j,
+                     p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
           }
         }
       }
     }
+    __syncthreads();
   }
-  a.cand[j] = best;
+  if (active) a.cand[j] = best;
 }
 
 // ---------------------------------------------------------------------------
@@ -799,7 +945,7 @@ hipError_t launch_sort(SortBuffers& b, const uint32_t* dN, uint32_t maxN, uint32
 
 hipError_t launch_first(const uint32_t* keys, const uint32_t* dN, uint32_t nb, uint32_t* first, uint32_t* dLive,
                         hipStream_t stream) {
-  hipLaunchKernelGGL(k_first, dim3(cdiv(cdiv(nb, 16), 256)), dim3(256), 0, stream, keys, dN, nb, first, dLive);
+  hipLaunchKernelGGL(k_first, dim3(cdiv(nb, kFirstSpan)), dim3(256), 0, stream, keys, dN, nb, first, dLive);
   return hipGetLastError();
 }
 

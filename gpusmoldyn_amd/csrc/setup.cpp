@@ -615,6 +615,18 @@ bool derive_tables(const smd_sim_spec& sp, ReplicaTables& T, smd_rxn_derived* de
       return false;
     }
 
+  // Pair-search filters derived from the final pair table.
+  for (int a = 0; a < NT; ++a) {
+    T.partnerMask[a] = 0;
+    T.maxBindR2[a] = 0.0f;
+    for (int b = 0; b < NT; ++b) {
+      const int r = T.pairRxn[a * kMaxTypes + b];
+      if (r < 0) continue;
+      T.partnerMask[a] |= 1u << b;
+      T.maxBindR2[a] = std::fmax(T.maxBindR2[a], T.bindR2[r]);
+    }
+  }
+
   // Geometry.
   for (int d = 0; d < 3; ++d) {
     T.lo[d] = sp.box_lo[d];

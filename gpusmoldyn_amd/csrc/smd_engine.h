@@ -29,6 +29,10 @@ class Engine {
                          uint32_t* nTotal, uint32_t* nLive);
   smd_status profile(int nsteps, double* us, int maxStages, char* names, int namesCap, int* nStages);
   smd_status particle_steps(unsigned long long* out);
+  // Replace the particles of a single-replica engine (setArray / reset).
+  smd_status set_particles(const float* pos4, const int* types, uint32_t n);
+  float4* current_pos() const { return dPos[cur]; }
+  int* current_type() const { return dType[cur]; }
 
   int device;
   int radixBits;
@@ -87,3 +91,8 @@ class Engine {
 };
 
 }  // namespace smd
+
+// The opaque C handle of include/gpusmoldyn.h.
+struct smd_engine {
+  smd::Engine* impl;
+};

@@ -45,6 +45,10 @@ struct ReplicaTables {
   float prodPos[kMaxRxn][6];
   // order-2 reaction id per (type, type); -1 = none (particleSystem.cpp:651-673)
   int32_t pairRxn[kMaxTypes * kMaxTypes];
+  // per type: bit t set if an order-2 reaction pairs it with type t, and the
+  // largest squared binding radius over those reactions (pair-search filters)
+  uint32_t partnerMask[kMaxTypes];
+  float maxBindR2[kMaxTypes];
   // geometry
   float lo[3], hi[3];
   float L[3], halfL[3];

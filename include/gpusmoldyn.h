@@ -189,6 +189,29 @@ smd_status smd_mt_generate(int device, const uint32_t* seeds, unsigned n_streams
 smd_status smd_sort_pairs(int device, uint32_t* keys, uint32_t* values, unsigned n, unsigned key_bits,
                           int radix_bits);
 
+/* ---- ParticleSystem (particleSystem.h:50-218) through C ----
+ * The C++ class is declared in particle_system.h; these wrappers expose it to
+ * C and foreign-function callers.  smd_ps_update returns 1 (true), 0 (false:
+ * extinction or more than MAX_PARTICLES) or a negative status. */
+typedef struct smd_ps smd_ps;
+struct SimParams;
+/* The reference constructor: ParticleSystem(numParticles, gridSize, false). */
+smd_ps* smd_ps_create_reference(unsigned num_particles, unsigned gx, unsigned gy, unsigned gz);
+smd_ps* smd_ps_create(const smd_sim_spec* spec, const float* pos4, const int* types, unsigned n, int device);
+void smd_ps_destroy(smd_ps* ps);
+int smd_ps_update(smd_ps* ps);
+/* config: 0 = CONFIG_RANDOM, 1 = CONFIG_GRID */
+smd_status smd_ps_reset(smd_ps* ps, int config);
+int smd_ps_num_particles(smd_ps* ps);
+/* getArray(POSITION): copies numParticles float4 (at most cap) */
+smd_status smd_ps_get_positions(smd_ps* ps, float* out4, unsigned cap);
+smd_status smd_ps_get_types(smd_ps* ps, int* out, unsigned cap);
+/* setArray(POSITION, data, start, count) */
+smd_status smd_ps_set_positions(smd_ps* ps, const float* data4, int start, int count);
+smd_status smd_ps_species_counts(smd_ps* ps, unsigned* counts);
+smd_status smd_ps_set_params(smd_ps* ps, const struct SimParams* params);
+smd_status smd_ps_get_params(smd_ps* ps, struct SimParams* params);
+
 #ifdef __cplusplus
 }
 #endif

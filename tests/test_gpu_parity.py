@@ -66,7 +66,7 @@ def w_c1_ref():
 def w_periodic_reversible():
     spec = configs.make_spec(
         D=[0.0, 1.0, 1.0, 0.5], dt=1e-3,
-        reactions=[dict(order=2, reactants=[1, 2], products=[3], rate=20.0),
+        reactions=[dict(order=2, reactants=[1, 2], products=[3], rate=1.0),
                    dict(order=1, reactants=[3], products=[1, 2], rate=50.0, pgem=0.2),
                    dict(order=1, reactants=[3], products=[], rate=5.0)],
         box_lo=[0.0] * 3, box_hi=[5.0] * 3, faces=[BC_PERIODIC] * 6, grid_bits=(4, 4, 4),
