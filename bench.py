@@ -141,6 +141,8 @@ def main():
     ap.add_argument("--config", default="c2")
     ap.add_argument("--radix-bits", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="c4: skip the one-replica-per-launch comparison")
+    ap.add_argument("--no-graphs", action="store_true", help="one launch sequence per step instead of graphs")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -150,6 +152,8 @@ def main():
     reps, steps, desc = build(args.config)
     steps = args.steps if args.steps is not None else steps
     e = make_engine(reps, args.radix_bits)
+    if args.no_graphs:
+        e.set_graphs(False)
     e.step(args.warmup)
     ps, secs = timed(e, steps)
     rl, step = roofline(e, reps)
@@ -170,6 +174,22 @@ def main():
         "roofline": rl,
         "step_roofline": step,
     }
+    if args.config == "c4" and not args.no_plain:
+        # the same 512-replica workload, one simulation per launch sequence
+        pps, psecs = 0, 0.0
+        for k, rep in enumerate(reps):
+            p = make_engine([rep], args.radix_bits)
+            p.set_graphs(False)
+            p.step(args.warmup)
+            a, b = timed(p, steps)
+            pps += a
+            psecs += b
+            p.close()
+            if k % 64 == 63:
+                print(f"[bench] plain replicas done: {k + 1}/{len(reps)}", file=sys.stderr, flush=True)
+        plain = pps / psecs
+        out["batched_vs_plain"] = {"batched": out["value"], "plain": plain, "ratio": out["value"] / plain,
+                                   "plain_seconds": psecs, "batched_seconds": secs}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(reps, args.cpu_budget)
     print(json.dumps(out), flush=True)

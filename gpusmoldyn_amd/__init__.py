@@ -93,6 +93,7 @@ def lib():
         L.smd_engine_sync.argtypes = [vp]
         L.smd_engine_counts.argtypes = [vp, u32p]
         L.smd_engine_particle_steps.argtypes = [vp, ctypes.POINTER(ctypes.c_ulonglong)]
+        L.smd_engine_set_graphs.argtypes = [vp, ctypes.c_int]
         L.smd_engine_num_replicas.argtypes = [vp]
         L.smd_engine_num_replicas.restype = ctypes.c_uint
         L.smd_engine_get.argtypes = [vp, ctypes.c_int, f32p, i32p, ctypes.c_uint, ctypes.POINTER(ctypes.c_uint)]
@@ -257,6 +258,9 @@ class Engine:
         out = np.zeros(R, np.uint32)
         _check(lib().smd_engine_counts(self._h, _ptr(out)), "smd_engine_counts")
         return out
+
+    def set_graphs(self, on):
+        _check(lib().smd_engine_set_graphs(self._h, int(bool(on))), "smd_engine_set_graphs")
 
     def particle_steps(self):
         v = ctypes.c_ulonglong(0)

@@ -187,6 +187,13 @@ smd_status smd_engine_counts(smd_engine* e, unsigned* counts) {
   return e->impl->counts(counts);
 }
 
+smd_status smd_engine_set_graphs(smd_engine* e, int on) {
+  if (!e) return SMD_ERR_INVALID;
+  e->impl->useGraphs = on != 0;
+  if (!on) e->impl->drop_graph();
+  return SMD_OK;
+}
+
 smd_status smd_engine_particle_steps(smd_engine* e, unsigned long long* out) {
   if (!e || !out) return SMD_ERR_INVALID;
   return e->impl->particle_steps(out);

@@ -42,6 +42,7 @@ static inline uint32_t round64(uint32_t x) { return (x + 63u) & ~63u; }
 Engine::Engine(int dev, int rbits) : device(dev), radixBits(rbits == 4 ? 4 : 8) {}
 
 Engine::~Engine() {
+  drop_graph();
   if (stream) {
     (void)hipSetDevice(device);
     (void)hipStreamSynchronize(stream);
@@ -299,6 +300,7 @@ void Engine::refresh_bound(bool block) {
 
 smd_status Engine::grow(uint64_t need) {
   HIPCHK(hipStreamSynchronize(stream));
+  drop_graph();
   uint64_t nc = std::max<uint64_t>(need, 2ull * cap);
   if (nc >= (1ull << 31)) {
     set_error("particle count exceeds the 2^31 capacity limit");
@@ -339,6 +341,72 @@ smd_status Engine::ensure_capacity() {
 
 smd_status Engine::enqueue_step(StageTimer* timer) {
   const uint32_t maxN = (uint32_t)std::min<uint64_t>(Nub, cap);
+  smd_status st = enqueue_kernels(maxN, timer);
+  if (st != SMD_OK) return st;
+  ++stepsDone;
+  if (anySplit) Nub = std::min<uint64_t>(2 * Nub, (uint64_t)cap);
+  return enqueue_readback();
+}
+
+// Asynchronous readback of the current total into the pinned ring.
+smd_status Engine::enqueue_readback() {
+  if (pendingCount == kRing) refresh_bound(true);
+  const int slot = (pendingHead + pendingCount) % kRing;
+  HIPCHK(hipMemcpyAsync(&pinned[slot], dN, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipEventRecord(ring[slot].ev, stream));
+  ring[slot].step = stepsDone;
+  ++pendingCount;
+  return SMD_OK;
+}
+
+void Engine::drop_graph() {
+  if (gexec) (void)hipGraphExecDestroy(gexec);
+  gexec = nullptr;
+}
+
+// Two consecutive steps captured as one hipGraph: after two steps every
+// ping-pong buffer is back in place, so the graph can be replayed as long as
+// the particle bound stays within the grid sizes it was captured with.
+smd_status Engine::step_graph(int pairs) {
+  for (int k = 0; k < pairs; ++k) {
+    refresh_bound(false);
+    uint64_t need = anySplit ? 4 * Nub : Nub;  // output of the second step
+    if (need > cap) {
+      refresh_bound(true);
+      need = anySplit ? 4 * Nub : Nub;
+      if (need > cap) {
+        drop_graph();
+        smd_status st = grow(2 * need);
+        if (st != SMD_OK) return st;
+      }
+    }
+    const uint64_t bound = anySplit ? 2 * Nub : Nub;  // input bound of the second step
+    if (gexec && (cur != gParity || bound > gMaxN)) drop_graph();
+    if (!gexec) {
+      const uint32_t maxN = (uint32_t)std::min<uint64_t>(bound, cap);
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      smd_status st = enqueue_kernels(maxN, nullptr);
+      if (st == SMD_OK) st = enqueue_kernels(maxN, nullptr);
+      hipError_t e = hipStreamEndCapture(stream, &g);
+      if (st != SMD_OK) return st;
+      HIPCHK(e);
+      e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HIPCHK(e);
+      gParity = cur;
+      gMaxN = maxN;
+    }
+    HIPCHK(hipGraphLaunch(gexec, stream));
+    stepsDone += 2;
+    if (anySplit) Nub = std::min<uint64_t>(4 * Nub, (uint64_t)cap);
+    smd_status st = enqueue_readback();
+    if (st != SMD_OK) return st;
+  }
+  return SMD_OK;
+}
+
+smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
   auto mark = [&](int stage) {
     if (timer) (void)hipEventRecord(timer->ev[stage], stream);
   };
@@ -442,15 +510,6 @@ smd_status Engine::enqueue_step(StageTimer* timer) {
   std::swap(dRepOff, dNewOff);
   std::swap(dRepCount, dNewCount);
   std::swap(dN, dNnext);
-  ++stepsDone;
-  if (anySplit) Nub = std::min<uint64_t>(2 * Nub, (uint64_t)cap);
-  // Asynchronous readback of the new total.
-  if (pendingCount == kRing) refresh_bound(true);
-  const int slot = (pendingHead + pendingCount) % kRing;
-  HIPCHK(hipMemcpyAsync(&pinned[slot], dN, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-  HIPCHK(hipEventRecord(ring[slot].ev, stream));
-  ring[slot].step = stepsDone;
-  ++pendingCount;
   return SMD_OK;
 }
 
@@ -464,6 +523,11 @@ smd_status Engine::step(int nsteps) {
     return SMD_ERR_INVALID;
   }
   HIPCHK(hipSetDevice(device));
+  if (useGraphs && nsteps >= 2) {
+    smd_status st = step_graph(nsteps / 2);
+    if (st != SMD_OK) return st;
+    nsteps -= 2 * (nsteps / 2);
+  }
   for (int s = 0; s < nsteps; ++s) {
     smd_status st = ensure_capacity();
     if (st != SMD_OK) return st;

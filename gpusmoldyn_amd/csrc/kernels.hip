@@ -11,6 +11,8 @@
 //   k_scatter     order-preserving compaction of survivors, births appended per replica
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "smd_device.h"
 #include "smd_kernels.h"
 
@@ -91,14 +93,28 @@ __global__ __launch_bounds__(kBS) void k_producer(ProducerArgs a) {
     Mt m;
     if (laneOK) mt_load<kBS>(m, &lds_mt[threadIdx.x], a.mtState, a.mtPar, a.Stot, g);
     const uint32_t iters = (n > s0) ? (n - s0 + S - 1) / S : 0;
+    // The next particle's position and type are requested before the
+    // current one is processed (two loads in flight per lane).
+    float4 pNext = make_float4(0.f, 0.f, 0.f, 0.f);
+    int tNext = 0;
+    if (laneOK && s < n) {
+      pNext = a.pos[off + s];
+      tNext = a.type[off + s];
+    }
     for (uint32_t it = 0; it < iters; ++it) {
       const uint32_t il = s + it * S;
       const bool act = laneOK && il < n;
       const uint32_t i = off + il;
+      const float4 pCur = pNext;
+      const int tCur = tNext;
+      if (laneOK && il + S < n) {
+        pNext = a.pos[i + S];
+        tNext = a.type[i + S];
+      }
       bool alive = false, birth = false;
       if (act) {
-        float4 p = a.pos[i];
-        int t = a.type[i];
+        float4 p = pCur;
+        int t = tCur;
         uint32_t u0, u1, u2;
         mt_draw3<kBS>(m, u0, u1, u2);
         const float rt = T.rate[t > 0 ? t : 0];
@@ -354,6 +370,179 @@ __global__ __launch_bounds__(256) void k_sort_down(const uint32_t* __restrict__ 
         s_tstart[d] = carry + ex;
       }
       carry += tot;
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < kSortItems; ++it) {
+    const uint32_t e = base + it * 64 + lane;
+    if (e < n) {
+      const uint32_t d = (kk[it] >> shift) & (RADIX - 1);
+      const uint32_t lp = s_tstart[d] + s_wcnt[w][d] + rk[it];
+      s_key[lp] = kk[it];
+      s_val[lp] = vv[it];
+    }
+  }
+  __syncthreads();
+  const uint32_t tileN = min((uint32_t)kSortTile, n - tile * kSortTile);
+  for (uint32_t idx = t; idx < tileN; idx += 256) {
+    const uint32_t k = s_key[idx];
+    const uint32_t d = (k >> shift) & (RADIX - 1);
+    const uint32_t dst = s_base[d] + (idx - s_tstart[d]);
+    keysOut[dst] = k;
+    valsOut[dst] = s_val[idx];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Single-sweep variant: one histogram kernel for all digit passes, then one
+// binning kernel per pass.  Tiles are numbered in the order workgroups start
+// (atomic counter); each tile publishes its per-digit count and finds the
+// exclusive prefix over earlier tiles by decoupled look-back on per-tile
+// status words (bit 31: inclusive prefix, bit 30: tile count only, bits
+// 0-29: value).  A tile waits only on tiles numbered before it, which are
+// already running, and every wait is bounded (error flag, no hang).
+// ---------------------------------------------------------------------------
+constexpr uint32_t kOsPrefix = 0x80000000u;
+constexpr uint32_t kOsAgg = 0x40000000u;
+constexpr uint32_t kOsValue = 0x3FFFFFFFu;
+constexpr uint32_t kOsSpinLimit = 1u << 22;
+
+__global__ __launch_bounds__(256) void k_os_zero(uint32_t* p, uint32_t n) {
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) p[i] = 0u;
+}
+
+template <int BITS>
+__global__ __launch_bounds__(256) void k_os_hist(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ dN,
+                                                 uint32_t maxN, int passes, uint32_t* __restrict__ hist) {
+  constexpr int RADIX = 1 << BITS;
+  __shared__ uint32_t h[(32 / BITS) * RADIX];
+  const uint32_t n = min(*dN, maxN);
+  for (int k = threadIdx.x; k < passes * RADIX; k += 256) h[k] = 0u;
+  __syncthreads();
+  for (uint32_t e = blockIdx.x * 256 + threadIdx.x; e < n; e += gridDim.x * 256) {
+    const uint32_t k = keys[e];
+    for (int p = 0; p < passes; ++p) atomicAdd(&h[p * RADIX + ((k >> (p * BITS)) & (RADIX - 1))], 1u);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < passes * RADIX; k += 256)
+    if (h[k]) atomicAdd(&hist[k], h[k]);
+}
+
+template <int BITS>
+__global__ __launch_bounds__(256) void k_os_pass(const uint32_t* __restrict__ keysIn,
+                                                 const uint32_t* __restrict__ valsIn, uint32_t* __restrict__ keysOut,
+                                                 uint32_t* __restrict__ valsOut, const uint32_t* __restrict__ dN,
+                                                 uint32_t maxN, const uint32_t* __restrict__ hist,
+                                                 uint32_t* status, uint32_t* tileCounter, uint32_t* err, int shift,
+                                                 int identityVals) {
+  constexpr int RADIX = 1 << BITS;
+  constexpr int DPT = (RADIX + 255) / 256;
+  __shared__ uint32_t s_base[RADIX];
+  __shared__ uint32_t s_wcnt[4][RADIX];
+  __shared__ uint32_t s_tstart[RADIX];
+  __shared__ uint32_t s_key[kSortTile];
+  __shared__ uint32_t s_val[kSortTile];
+  __shared__ uint32_t wsum[4];
+  __shared__ uint32_t tot;
+  __shared__ uint32_t s_tile;
+
+  const uint32_t n = min(*dN, maxN);
+  const uint32_t nTiles = (n + kSortTile - 1) / kSortTile;
+  if (threadIdx.x == 0) s_tile = atomicAdd(tileCounter, 1u);
+  const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+  for (uint32_t d = t; d < (uint32_t)RADIX; d += 256) {
+    s_wcnt[0][d] = 0;
+    s_wcnt[1][d] = 0;
+    s_wcnt[2][d] = 0;
+    s_wcnt[3][d] = 0;
+  }
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  if (tile >= nTiles) return;
+
+  // local ranks (wave64 ballot match per 64-element chunk, element order)
+  const uint32_t base = tile * kSortTile + w * (64 * kSortItems);
+  uint32_t kk[kSortItems], vv[kSortItems], rk[kSortItems];
+  const uint64_t lt = lanemask_lt();
+#pragma unroll
+  for (int it = 0; it < kSortItems; ++it) {
+    const uint32_t e = base + it * 64 + lane;
+    const bool valid = e < n;
+    const uint32_t k = valid ? keysIn[e] : 0u;
+    kk[it] = k;
+    vv[it] = identityVals ? e : (valid ? valsIn[e] : 0u);
+    const uint32_t d = (k >> shift) & (RADIX - 1);
+    uint64_t mm = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < BITS; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bb = __ballot(bit);
+      mm &= bit ? bb : ~bb;
+    }
+    const uint32_t prev = s_wcnt[w][d];
+    rk[it] = prev + (uint32_t)__popcll(mm & lt);
+    if (valid && (mm & lt) == 0ull) s_wcnt[w][d] = prev + (uint32_t)__popcll(mm);
+  }
+  __syncthreads();
+  // per-digit wave offsets, tile counts, look-back, global bases
+  {
+    uint32_t carry = 0, gcarry = 0;
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+      const uint32_t d = q * 256 + t;
+      const bool dv = d < (uint32_t)RADIX;
+      uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0, g = 0;
+      if (dv) {
+        c0 = s_wcnt[0][d];
+        c1 = s_wcnt[1][d];
+        c2 = s_wcnt[2][d];
+        c3 = s_wcnt[3][d];
+        g = hist[d];
+      }
+      const uint32_t cnt = c0 + c1 + c2 + c3;
+      uint32_t excl = 0;
+      if (dv) {
+        uint32_t* st = status + (size_t)tile * RADIX + d;
+        if (tile == 0) {
+          __hip_atomic_store(st, kOsPrefix | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+          __hip_atomic_store(st, kOsAgg | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          int32_t tp = (int32_t)tile - 1;
+          uint32_t spins = 0;
+          while (tp >= 0) {
+            const uint32_t v =
+                __hip_atomic_load(status + (size_t)tp * RADIX + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((v & (kOsPrefix | kOsAgg)) == 0u) {
+              if (++spins > kOsSpinLimit) {
+                atomicOr(err, 1u);
+                break;
+              }
+              __builtin_amdgcn_s_sleep(1);
+              continue;
+            }
+            excl += v & kOsValue;
+            if (v & kOsPrefix) break;
+            --tp;
+          }
+          __hip_atomic_store(st, kOsPrefix | (excl + cnt), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+      const uint32_t ex = block_exscan256(cnt, wsum, &tot);
+      const uint32_t tt = tot;
+      __syncthreads();
+      const uint32_t gex = block_exscan256(g, wsum, &tot);
+      const uint32_t gt = tot;
+      if (dv) {
+        s_wcnt[0][d] = 0;
+        s_wcnt[1][d] = c0;
+        s_wcnt[2][d] = c0 + c1;
+        s_wcnt[3][d] = c0 + c1 + c2;
+        s_tstart[d] = carry + ex;
+        s_base[d] = gcarry + gex + excl;
+      }
+      carry += tt;
+      gcarry += gt;
       __syncthreads();
     }
   }
@@ -906,11 +1095,59 @@ hipError_t launch_producer(const ProducerArgs& a, hipStream_t stream) {
 int sort_passes(uint32_t keyBits, int radixBits) { return (int)((keyBits + radixBits - 1) / radixBits); }
 
 size_t sort_counts_words(uint32_t maxN, int radixBits) {
-  return (size_t)(1u << radixBits) * (cdiv(maxN, kSortTile) + 1) + (1u << radixBits);
+  // enough for both layouts: per-tile counters of the three-kernel sort, or
+  // histograms + tile counters + error flag + per-pass status words
+  const size_t radix = (size_t)1 << radixBits;
+  const size_t tiles = cdiv(maxN, kSortTile) + 1;
+  const size_t passes = (size_t)((32 + radixBits - 1) / radixBits);
+  const size_t a = radix * tiles + radix;
+  const size_t b = passes * radix + passes + 4 + passes * tiles * radix;
+  return a > b ? a : b;
+}
+
+static hipError_t launch_sort_onesweep(SortBuffers& b, const uint32_t* dN, uint32_t maxN, uint32_t keyBits,
+                                       int radixBits, bool identityVals, hipStream_t stream) {
+  const uint32_t maxTiles = cdiv(maxN, kSortTile);
+  if (maxTiles == 0) return hipSuccess;
+  const int radix = 1 << radixBits;
+  const int passes = sort_passes(keyBits, radixBits);
+  uint32_t* hist = b.counts;
+  uint32_t* counters = hist + (size_t)passes * radix;
+  uint32_t* err = counters + passes;
+  uint32_t* status = err + 4;
+  const uint32_t zeroWords = (uint32_t)((size_t)passes * radix + passes + 4 + (size_t)passes * maxTiles * radix);
+  hipLaunchKernelGGL(k_os_zero, dim3(std::min<unsigned>(cdiv(zeroWords, 256), 1024)), dim3(256), 0, stream, b.counts,
+                     zeroWords);
+  const uint32_t histBlocks = std::min<uint32_t>(cdiv(maxN, 16384), 1024);
+  if (radixBits == 8)
+    hipLaunchKernelGGL(k_os_hist<8>, dim3(histBlocks), dim3(256), 0, stream, b.keys, dN, maxN, passes, hist);
+  else if (radixBits == 4)
+    hipLaunchKernelGGL(k_os_hist<4>, dim3(histBlocks), dim3(256), 0, stream, b.keys, dN, maxN, passes, hist);
+  else
+    return hipErrorInvalidValue;
+  for (int p = 0; p < passes; ++p) {
+    const int shift = p * radixBits;
+    const int idv = (identityVals && p == 0) ? 1 : 0;
+    uint32_t* st = status + (size_t)p * maxTiles * radix;
+    if (radixBits == 8)
+      hipLaunchKernelGGL(k_os_pass<8>, dim3(maxTiles), dim3(256), 0, stream, b.keys, b.vals, b.keysAlt, b.valsAlt, dN,
+                         maxTiles * (uint32_t)kSortTile, hist + (size_t)p * radix, st, counters + p, err, shift, idv);
+    else
+      hipLaunchKernelGGL(k_os_pass<4>, dim3(maxTiles), dim3(256), 0, stream, b.keys, b.vals, b.keysAlt, b.valsAlt, dN,
+                         maxTiles * (uint32_t)kSortTile, hist + (size_t)p * radix, st, counters + p, err, shift, idv);
+    std::swap(b.keys, b.keysAlt);
+    std::swap(b.vals, b.valsAlt);
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_sort(SortBuffers& b, const uint32_t* dN, uint32_t maxN, uint32_t keyBits, int radixBits,
                        bool identityVals, hipStream_t stream) {
+  return launch_sort_onesweep(b, dN, maxN, keyBits, radixBits, identityVals, stream);
+}
+
+hipError_t launch_sort_3k(SortBuffers& b, const uint32_t* dN, uint32_t maxN, uint32_t keyBits, int radixBits,
+                          bool identityVals, hipStream_t stream) {
   const uint32_t maxTiles = cdiv(maxN, kSortTile);
   if (maxTiles == 0) return hipSuccess;
   const uint32_t stride = maxTiles + 1;

@@ -81,7 +81,17 @@ class Engine {
   uint32_t* pinned = nullptr;
   int pendingHead = 0, pendingCount = 0;
 
+  // hipGraph replay of two-step pairs
+  bool useGraphs = true;
+  hipGraphExec_t gexec = nullptr;
+  int gParity = 0;
+  uint32_t gMaxN = 0;
+  void drop_graph();
+
  private:
+  smd_status enqueue_kernels(uint32_t maxN, StageTimer* timer);
+  smd_status enqueue_readback();
+  smd_status step_graph(int pairs);
   smd_status alloc_particle_buffers(uint32_t c);
   void free_particle_buffers();
   void refresh_bound(bool block);

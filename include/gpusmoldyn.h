@@ -148,6 +148,9 @@ smd_status smd_engine_finalize(smd_engine* e, unsigned capacity);
 /* Advance every replica by `nsteps` timesteps.  Asynchronous with respect to
  * the host except when capacity must be checked. */
 smd_status smd_engine_step(smd_engine* e, int nsteps);
+/* Replay pairs of steps as one captured hipGraph (default on); off = one
+ * launch sequence per step. */
+smd_status smd_engine_set_graphs(smd_engine* e, int on);
 smd_status smd_engine_sync(smd_engine* e);
 /* Current particle count of each replica (synchronises). */
 smd_status smd_engine_counts(smd_engine* e, unsigned* counts);
