@@ -1,12 +1,16 @@
 """Benchmark of the gpusmoldyn timestep on one MI355X.
 
-  python bench.py [--steps K] [--warmup W] [--config c2|c3|c4|c1]
+  python bench.py [--steps K] [--warmup W] [--config c2|c3|c4|c1|c5]
 
 A step is one full timestep (diffuse + unimolecular reactions + boundaries +
 hash + sort + cell table + pair search + bimolecular reactions + compaction)
 of every particle of the workload, with the particle state resident in HBM.
 `value` is particle-steps/s = (sum over timed steps of the particles entering
 the step) / wall time.  Prints one JSON line.
+
+c5 is the serving workload: jobs arriving as a Poisson stream are admitted
+into replica slots and stepped together; `value` is then the aggregate
+particle-steps/s of the whole run and `serving` holds p50/p99 job latency.
 """
 import argparse
 import json
@@ -133,6 +137,38 @@ def cpu_baseline(reps, budget_s=20.0):
                       f"workload ({len(types)} particles), {t_used:.1f} s"}
 
 
+def serve(args):
+    from gpusmoldyn_amd import serving
+
+    jobs = serving.c5_jobs(n_jobs=args.jobs)
+    srv = serving.Server(n_slots=32, slot_bucket_bits=18, slot_streams=32768, capacity=1 << 24)
+    # warm-up: one small job through the same slots
+    srv.run([serving.make_job(-1, 0.0, 4096, 20, 3, 1)], realtime=False)
+    results, st = srv.run(jobs, realtime=True)
+    srv.close()
+    n_ps = st["particle_steps"]
+    return {
+        "metric": METRIC,
+        "value": st["particle_steps_per_s"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": st["engine_steps"],
+        "warmup": 1,
+        "ms_per_step": st["wall_s"] * 1e3 / max(st["engine_steps"], 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"c5: {len(jobs)} jobs, Poisson 20 jobs/s, n log-uniform 1k-256k, steps log-uniform "
+                               "100-10k, 2-4 species, random A+B<->C / decay tables; 32 slots of 2^18 buckets",
+                   "jobs": len(jobs), "particle_steps": n_ps},
+        "serving": {"p50_latency_s": st["p50_s"], "p99_latency_s": st["p99_s"], "wall_s": st["wall_s"],
+                    "jobs": st["jobs"], "last_arrival_s": jobs[-1].arrival},
+        "roofline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,11 +180,15 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="c4: skip the one-replica-per-launch comparison")
     ap.add_argument("--no-graphs", action="store_true", help="one launch sequence per step instead of graphs")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--jobs", type=int, default=200, help="c5: number of jobs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); replicas and slabs would shard "
-              "across GPUs as described in DESIGN.md, but this build runs on a single MI355X.", file=sys.stderr)
+              "across GPUs (a replica set or a spatial slab per GPU), but this build runs on a single MI355X.", file=sys.stderr)
         sys.exit(2)
+    if args.config == "c5":
+        print(json.dumps(serve(args)), flush=True)
+        return
     reps, steps, desc = build(args.config)
     steps = args.steps if args.steps is not None else steps
     e = make_engine(reps, args.radix_bits)

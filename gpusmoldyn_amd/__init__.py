@@ -89,6 +89,9 @@ def lib():
         L.smd_engine_destroy.argtypes = [vp]
         L.smd_engine_add_replica.argtypes = [vp, ctypes.POINTER(SimSpec), f32p, i32p, ctypes.c_uint]
         L.smd_engine_finalize.argtypes = [vp, ctypes.c_uint]
+        L.smd_engine_finalize_slots.argtypes = [vp] + [ctypes.c_uint] * 5
+        L.smd_engine_admit.argtypes = [vp, ctypes.c_int, ctypes.POINTER(SimSpec), f32p, i32p, ctypes.c_uint]
+        L.smd_engine_retire.argtypes = [vp, ctypes.c_int]
         L.smd_engine_step.argtypes = [vp, ctypes.c_int]
         L.smd_engine_sync.argtypes = [vp]
         L.smd_engine_counts.argtypes = [vp, u32p]
@@ -246,6 +249,24 @@ class Engine:
     def finalize(self, capacity=0):
         _check(lib().smd_engine_finalize(self._h, capacity), "smd_engine_finalize")
         self.finalized = True
+
+    def finalize_slots(self, n_slots, slot_bucket_bits, slot_streams, capacity, inject_cap):
+        """Serving layout: n_slots empty replica slots (see admit/retire)."""
+        _check(lib().smd_engine_finalize_slots(self._h, n_slots, slot_bucket_bits, slot_streams, capacity,
+                                               inject_cap), "smd_engine_finalize_slots")
+        self.specs = [None] * n_slots
+        self.finalized = True
+
+    def admit(self, slot, spec, pos4, types):
+        pos4 = np.ascontiguousarray(pos4, np.float32).reshape(-1, 4)
+        types = np.ascontiguousarray(types, np.int32)
+        assert len(pos4) == len(types)
+        _check(lib().smd_engine_admit(self._h, slot, ctypes.byref(spec), _ptr(pos4), _ptr(types), len(types)),
+               "smd_engine_admit")
+        self.specs[slot] = spec
+
+    def retire(self, slot):
+        _check(lib().smd_engine_retire(self._h, slot), "smd_engine_retire")
 
     def step(self, nsteps=1):
         _check(lib().smd_engine_step(self._h, nsteps), "smd_engine_step")

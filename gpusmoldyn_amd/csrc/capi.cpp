@@ -172,6 +172,23 @@ smd_status smd_engine_finalize(smd_engine* e, unsigned capacity) {
   return e->impl->finalize(capacity);
 }
 
+smd_status smd_engine_finalize_slots(smd_engine* e, unsigned n_slots, unsigned slot_bucket_bits,
+                                     unsigned slot_streams, unsigned capacity, unsigned inject_cap) {
+  if (!e) return SMD_ERR_INVALID;
+  return e->impl->finalize_slots(n_slots, slot_bucket_bits, slot_streams, capacity, inject_cap);
+}
+
+smd_status smd_engine_admit(smd_engine* e, int slot, const smd_sim_spec* spec, const float* pos4, const int* types,
+                            unsigned n) {
+  if (!e || !spec || (n && (!pos4 || !types))) return SMD_ERR_INVALID;
+  return e->impl->admit(slot, *spec, pos4, types, n);
+}
+
+smd_status smd_engine_retire(smd_engine* e, int slot) {
+  if (!e) return SMD_ERR_INVALID;
+  return e->impl->retire(slot);
+}
+
 smd_status smd_engine_step(smd_engine* e, int nsteps) {
   if (!e || nsteps < 0) return SMD_ERR_INVALID;
   return e->impl->step(nsteps);

@@ -66,6 +66,13 @@ Engine::~Engine() {
   dfree(dMtPar);
   dfree(dGauss);
   dfree(dFirst);
+  dfree(dRetire);
+  dfree(dInjCount);
+  dfree(dInjUse);
+  dfree(dInjSrcOff);
+  dfree(dInjPos);
+  dfree(dInjType);
+  dfree(dInjRep);
   for (auto& ev : ring) (void)hipEventDestroy(ev.ev);
   if (pinned) (void)hipHostFree(pinned);
   if (stream) (void)hipStreamDestroy(stream);
@@ -430,6 +437,7 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
   pa.repBirths = dRepBirths;
   pa.tileAlive = dTileAlive;
   pa.tileBirth = dTileBirth;
+  pa.retire = dRetire;
   HIPCHK(launch_producer(pa, stream));
   mark(1);
   if (anyBim) {
@@ -486,6 +494,9 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
   fa.tileAlivePre = dTileAlivePre;
   fa.tileBirthPre = dTileBirthPre;
   fa.particleSteps = dParticleSteps;
+  fa.injCount = dInjCount;
+  fa.injUse = dInjUse;
+  fa.retire = dRetire;
   HIPCHK(launch_finalize(fa, stream));
   mark(7);
   ScatterArgs sa;
@@ -505,6 +516,22 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
   sa.posOut = dPos[cur ^ 1];
   sa.typeOut = dType[cur ^ 1];
   HIPCHK(launch_scatter(sa, maxN, stream));
+  if (injN) {
+    InjectArgs ia;
+    ia.injPos = dInjPos;
+    ia.injType = dInjType;
+    ia.injRep = dInjRep;
+    ia.injSrcOff = dInjSrcOff;
+    ia.nInj = injN;
+    ia.newOff = dNewOff;
+    ia.newCount = dNewCount;
+    ia.repAlive = dRepAlive;
+    ia.injUse = dInjUse;
+    ia.posOut = dPos[cur ^ 1];
+    ia.typeOut = dType[cur ^ 1];
+    HIPCHK(launch_inject(ia, stream));
+    injN = 0;
+  }
   mark(8);
   cur ^= 1;
   std::swap(dRepOff, dNewOff);
@@ -523,6 +550,14 @@ smd_status Engine::step(int nsteps) {
     return SMD_ERR_INVALID;
   }
   HIPCHK(hipSetDevice(device));
+  if (injN && nsteps > 0) {
+    // admitted particles enter through an eager step (graphs hold no injection)
+    smd_status st = ensure_capacity();
+    if (st != SMD_OK) return st;
+    st = enqueue_step(nullptr);
+    if (st != SMD_OK) return st;
+    --nsteps;
+  }
   if (useGraphs && nsteps >= 2) {
     smd_status st = step_graph(nsteps / 2);
     if (st != SMD_OK) return st;
@@ -541,6 +576,149 @@ smd_status Engine::sync() {
   HIPCHK(hipSetDevice(device));
   HIPCHK(hipStreamSynchronize(stream));
   refresh_bound(true);
+  return SMD_OK;
+}
+
+// Serving layout: nSlots replicas with fixed bucket ranges (2^slotBucketBits
+// each) and stream ranges (slotStreams each), all empty at first.
+smd_status Engine::finalize_slots(uint32_t nSlots, uint32_t slotBucketBits, uint32_t slotStreamsMax,
+                                  uint32_t capacity, uint32_t injectCap) {
+  if (finalized || !tabs.empty()) {
+    set_error("finalize_slots needs a fresh engine");
+    return SMD_ERR_INVALID;
+  }
+  if (nSlots == 0 || slotBucketBits < 6 || slotBucketBits > 24 || slotStreamsMax == 0 ||
+      ((uint64_t)nSlots << slotBucketBits) >= (1ull << 31)) {
+    set_error("bad slot layout");
+    return SMD_ERR_INVALID;
+  }
+  // Placeholder replicas: no particles, no streams, no reactions.
+  smd_sim_spec s;
+  std::memset(&s, 0, sizeof(s));
+  s.num_types = 2;
+  s.dt = 1.0f;
+  for (int d = 0; d < 3; ++d) {
+    s.box_hi[d] = 1.0f;
+    s.grid_bits[d] = 2;
+  }
+  for (int f = 0; f < 6; ++f) s.face[f] = kReflect;
+  s.num_streams = slotStreamsMax;
+  for (uint32_t r = 0; r < nSlots; ++r) {
+    const int id = add_replica(s, nullptr, nullptr, 0);
+    if (id < 0) return (smd_status)(-id);
+    tabs[r].numBuckets = 1u << slotBucketBits;  // reserve the whole range
+  }
+  slotsMode = true;
+  slotBits = slotBucketBits;
+  slotStreams = slotStreamsMax;
+  injCap = injectCap;
+  smd_status st = finalize(capacity);
+  if (st != SMD_OK) return st;
+  anySplit = true;  // jobs may create particles
+  anyBim = true;
+  // placeholders own no streams
+  std::vector<int> none(Stot, -1);
+  HIPCHK(hipMemcpy(dStreamReplica, none.data(), Stot * sizeof(int), hipMemcpyHostToDevice));
+  for (uint32_t r = 0; r < R; ++r) tabs[r].numStreams = 0;
+  HIPCHK(hipMemcpy(dTables, tabs.data(), R * sizeof(ReplicaTables), hipMemcpyHostToDevice));
+  for (uint32_t** p : {&dRetire, &dInjCount, &dInjUse, &dInjSrcOff}) {
+    HIPCHK(dalloc(p, R));
+    HIPCHK(hipMemset(*p, 0, R * sizeof(uint32_t)));
+  }
+  HIPCHK(dalloc(&dInjPos, injCap));
+  HIPCHK(dalloc(&dInjType, injCap));
+  HIPCHK(dalloc(&dInjRep, injCap));
+  slotBusy.assign(R, 0);
+  return SMD_OK;
+}
+
+smd_status Engine::admit(int slot, const smd_sim_spec& spec, const float* pos4, const int* types, uint32_t n) {
+  if (!slotsMode || slot < 0 || (uint32_t)slot >= R) {
+    set_error("admit needs a slot engine and a valid slot");
+    return SMD_ERR_INVALID;
+  }
+  if (slotBusy[slot]) {
+    set_error("slot is occupied (retire its job first)");
+    return SMD_ERR_INVALID;
+  }
+  if (injN + n > injCap) {
+    set_error("admission exceeds the injection buffer of this step");
+    return SMD_ERR_CAPACITY;
+  }
+  ReplicaTables T;
+  std::string err;
+  std::vector<smd_rxn_derived> der(spec.num_reactions > 0 ? spec.num_reactions : 1);
+  if (!derive_tables(spec, T, der.data(), err)) {
+    set_error(err);
+    return SMD_ERR_RXN_SETUP;
+  }
+  if (T.gbits[0] + T.gbits[1] + T.gbits[2] > slotBits || T.numStreams > slotStreams) {
+    set_error("job grid or stream count exceeds the slot");
+    return SMD_ERR_INVALID;
+  }
+  for (uint32_t i = 0; i < n; ++i)
+    if (types[i] < 1 || types[i] >= spec.num_types) {
+      set_error("particle type out of range");
+      return SMD_ERR_INVALID;
+    }
+  smd_status st = sync();
+  if (st != SMD_OK) return st;
+  const uint32_t S = T.numStreams;
+  T.bucketOff = tabs[slot].bucketOff;
+  T.streamOff = tabs[slot].streamOff;
+  tabs[slot] = T;
+  specs[slot] = spec;
+  derived[slot] = der;
+  HIPCHK(hipMemcpy(dTables + slot, &T, sizeof(ReplicaTables), hipMemcpyHostToDevice));
+  std::vector<int> rep(round64(slotStreams), -1);
+  for (uint32_t k = 0; k < round64(S); ++k) rep[k] = slot;
+  HIPCHK(hipMemcpy(dStreamReplica + T.streamOff, rep.data(), rep.size() * sizeof(int), hipMemcpyHostToDevice));
+  const uint32_t* dat = mt_dat(err);
+  if (!dat) {
+    set_error(err);
+    return SMD_ERR_IO;
+  }
+  std::vector<uint32_t> seeds(S), par(S);
+  if (!stream_seeds(spec.seed, S, seeds.data(), err)) {
+    set_error(err);
+    return SMD_ERR_IO;
+  }
+  for (int w = 0; w < 3; ++w) {
+    for (uint32_t k = 0; k < S; ++k) par[k] = dat[4 * (k % kMtParamSets) + w];
+    HIPCHK(hipMemcpy(dMtPar + (size_t)w * Stot + T.streamOff, par.data(), S * sizeof(uint32_t),
+                     hipMemcpyHostToDevice));
+  }
+  uint32_t* dSeeds = nullptr;
+  HIPCHK(dalloc(&dSeeds, S));
+  HIPCHK(hipMemcpy(dSeeds, seeds.data(), S * sizeof(uint32_t), hipMemcpyHostToDevice));
+  HIPCHK(launch_mt_init_range(dMtState, dSeeds, Stot, T.streamOff, S, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  dfree(dSeeds);
+  // particles enter during the next step's compaction
+  if (n) {
+    HIPCHK(hipMemcpy(dInjPos + injN, pos4, n * sizeof(float4), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dInjType + injN, types, n * sizeof(int), hipMemcpyHostToDevice));
+    std::vector<int> r(n, slot);
+    HIPCHK(hipMemcpy(dInjRep + injN, r.data(), n * sizeof(int), hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipMemcpy(dInjSrcOff + slot, &injN, sizeof(uint32_t), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dInjCount + slot, &n, sizeof(uint32_t), hipMemcpyHostToDevice));
+  injN += n;
+  Nub += n;
+  slotBusy[slot] = 1;
+  return SMD_OK;
+}
+
+smd_status Engine::retire(int slot) {
+  if (!slotsMode || slot < 0 || (uint32_t)slot >= R) {
+    set_error("retire needs a slot engine and a valid slot");
+    return SMD_ERR_INVALID;
+  }
+  smd_status st = sync();
+  if (st != SMD_OK) return st;
+  const uint32_t one = 1u;
+  HIPCHK(hipMemcpy(dRetire + slot, &one, sizeof(uint32_t), hipMemcpyHostToDevice));
+  slotBusy[slot] = 0;
   return SMD_OK;
 }
 
@@ -660,6 +838,7 @@ smd_status Engine::debug_stage(int stage, float* pos4, int* types, uint32_t* key
   pa.repBirths = dRepBirths;
   pa.tileAlive = dTileAlive;
   pa.tileBirth = dTileBirth;
+  pa.retire = nullptr;
   HIPCHK(launch_producer(pa, stream));
   uint32_t live = 0;
   if (stage >= 2) {

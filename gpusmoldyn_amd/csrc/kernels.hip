@@ -90,8 +90,11 @@ __global__ __launch_bounds__(kBS) void k_producer(ProducerArgs a) {
     const uint32_t n = a.repCount[rw];
     const uint32_t off = a.repOff[rw];
     const uint32_t bucketOff = T.bucketOff;
+    // A retiring replica (a finished serving job) has all its particles
+    // removed this step, without drawing.
+    const bool kill = a.retire != nullptr && a.retire[rw] != 0u;
     Mt m;
-    if (laneOK) mt_load<kBS>(m, &lds_mt[threadIdx.x], a.mtState, a.mtPar, a.Stot, g);
+    if (laneOK && !kill) mt_load<kBS>(m, &lds_mt[threadIdx.x], a.mtState, a.mtPar, a.Stot, g);
     const uint32_t iters = (n > s0) ? (n - s0 + S - 1) / S : 0;
     // The next particle's position and type are requested before the
     // current one is processed (two loads in flight per lane).
@@ -112,7 +115,12 @@ __global__ __launch_bounds__(kBS) void k_producer(ProducerArgs a) {
         tNext = a.type[i + S];
       }
       bool alive = false, birth = false;
-      if (act) {
+      if (act && kill) {
+        a.type[i] = 0;
+        a.key[i] = a.deadKey;
+        a.birthType[i] = 0;
+        ++nDeaths;
+      } else if (act) {
         float4 p = pCur;
         int t = tCur;
         uint32_t u0, u1, u2;
@@ -198,7 +206,7 @@ __global__ __launch_bounds__(kBS) void k_producer(ProducerArgs a) {
         if (bHi) atomicAdd(&a.tileBirth[t0 + 1], bHi);
       }
     }
-    if (laneOK) mt_store<kBS>(m, a.mtState, a.Stot, g);
+    if (laneOK && !kill) mt_store<kBS>(m, a.mtState, a.Stot, g);
   }
   nDeaths = wave_sum(nDeaths);
   nBirths = wave_sum(nBirths);
@@ -944,12 +952,13 @@ __global__ __launch_bounds__(1024) void k_finalize(FinalizeArgs a) {
   uint32_t cN = 0, cA = 0, cB = 0;
   for (uint32_t b = 0; b < (uint32_t)a.R; b += 1024) {
     const uint32_t r = b + threadIdx.x;
-    uint32_t alive = 0, births = 0;
+    uint32_t alive = 0, births = 0, inj = 0;
     if (r < (uint32_t)a.R) {
       alive = a.repCount[r] - a.repDeaths[r];
       births = a.repBirths[r];
+      if (a.injCount) inj = a.injCount[r];
     }
-    const uint32_t exN = block_exscan1024(alive + births, wsum, &tot);
+    const uint32_t exN = block_exscan1024(alive + births + inj, wsum, &tot);
     const uint32_t tN = tot;
     __syncthreads();
     const uint32_t exA = block_exscan1024(alive, wsum, &tot);
@@ -959,12 +968,17 @@ __global__ __launch_bounds__(1024) void k_finalize(FinalizeArgs a) {
     const uint32_t tB = tot;
     if (r < (uint32_t)a.R) {
       a.newOff[r] = cN + exN;
-      a.newCount[r] = alive + births;
+      a.newCount[r] = alive + births + inj;
       a.aliveBase[r] = cA + exA;
       a.birthBase[r] = cB + exB;
       a.repAlive[r] = alive;
       a.repDeaths[r] = 0;
       a.repBirths[r] = 0;
+      if (a.injCount) {
+        a.injUse[r] = inj;
+        a.injCount[r] = 0;
+      }
+      if (a.retire) a.retire[r] = 0;
     }
     cN += tN;
     cA += tA;
@@ -1066,6 +1080,35 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
       a.typeOut[dst] = a.birthType[e];
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Serving: admitted particles enter their replica's segment after its births;
+// stream ranges of an admitted replica are (re)initialised; a finished
+// replica's particles are removed by the producer.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_inject(InjectArgs a) {
+  const uint32_t q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= a.nInj) return;
+  const int r = a.injRep[q];
+  const uint32_t births = a.newCount[r] - a.repAlive[r] - a.injUse[r];
+  const uint32_t dst = a.newOff[r] + a.repAlive[r] + births + (q - a.injSrcOff[r]);
+  a.posOut[dst] = a.injPos[q];
+  a.typeOut[dst] = a.injType[q];
+}
+
+__global__ __launch_bounds__(256) void k_mt_init_range(uint32_t* __restrict__ st, const uint32_t* __restrict__ seeds,
+                                                       uint32_t Stot, uint32_t off, uint32_t count) {
+  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= count) return;
+  const uint32_t g = off + k;
+  uint32_t v = seeds[k];
+  st[g] = v;
+  for (int i = 1; i < kMtNN; ++i) {
+    v = 1812433253u * (v ^ (v >> 30)) + (uint32_t)i;
+    st[(size_t)i * Stot + g] = v;
+  }
+  st[(size_t)kMtNN * Stot + g] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -1209,6 +1252,20 @@ hipError_t launch_finalize(const FinalizeArgs& a, hipStream_t stream) {
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, stream, a);
   return hipGetLastError();
 }
+
+hipError_t launch_inject(const InjectArgs& a, hipStream_t stream) {
+  if (a.nInj == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_inject, dim3(cdiv(a.nInj, 256)), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_mt_init_range(uint32_t* st, const uint32_t* seeds, uint32_t Stot, uint32_t off, uint32_t count,
+                                hipStream_t stream) {
+  if (count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mt_init_range, dim3(cdiv(count, 256)), dim3(256), 0, stream, st, seeds, Stot, off, count);
+  return hipGetLastError();
+}
+
 
 hipError_t launch_scatter(const ScatterArgs& a, uint32_t maxN, hipStream_t stream) {
   if (maxN == 0) return hipSuccess;

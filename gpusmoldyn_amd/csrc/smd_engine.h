@@ -81,6 +81,20 @@ class Engine {
   uint32_t* pinned = nullptr;
   int pendingHead = 0, pendingCount = 0;
 
+  // Serving: fixed replica slots filled and emptied between steps.
+  smd_status finalize_slots(uint32_t nSlots, uint32_t slotBucketBits, uint32_t slotStreams, uint32_t capacity,
+                            uint32_t injectCap);
+  smd_status admit(int slot, const smd_sim_spec& spec, const float* pos4, const int* types, uint32_t n);
+  smd_status retire(int slot);
+  bool slotsMode = false;
+  uint32_t slotBits = 0, slotStreams = 0;
+  uint32_t *dRetire = nullptr, *dInjCount = nullptr, *dInjUse = nullptr, *dInjSrcOff = nullptr;
+  float4* dInjPos = nullptr;
+  int* dInjType = nullptr;
+  int* dInjRep = nullptr;
+  uint32_t injCap = 0, injN = 0;
+  std::vector<uint8_t> slotBusy;  // admitted and not yet retired
+
   // hipGraph replay of two-step pairs
   bool useGraphs = true;
   hipGraphExec_t gexec = nullptr;

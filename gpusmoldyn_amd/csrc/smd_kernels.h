@@ -26,6 +26,7 @@ struct ProducerArgs {
   uint32_t* repBirths;
   uint32_t* tileAlive;
   uint32_t* tileBirth;
+  const uint32_t* retire;     // per replica: remove every particle this step (nullptr: none)
 };
 
 struct SortBuffers {
@@ -80,6 +81,23 @@ struct FinalizeArgs {
   uint32_t* tileAlivePre;
   uint32_t* tileBirthPre;
   unsigned long long* particleSteps;  // += particles entering the step
+  uint32_t* injCount;  // per replica: particles admitted this step (nullptr: none); reset
+  uint32_t* injUse;    // per replica: copy of injCount for the injection kernel
+  uint32_t* retire;    // per replica retire flags (nullptr: none); reset
+};
+
+struct InjectArgs {
+  const float4* injPos;
+  const int* injType;
+  const int* injRep;          // replica of each admitted particle
+  const uint32_t* injSrcOff;  // first admitted particle of each replica
+  uint32_t nInj;
+  const uint32_t* newOff;
+  const uint32_t* newCount;
+  const uint32_t* repAlive;
+  const uint32_t* injUse;
+  float4* posOut;
+  int* typeOut;
 };
 
 struct ScatterArgs {
@@ -116,5 +134,8 @@ hipError_t launch_candidates(const CandArgs& a, uint32_t maxN, hipStream_t strea
 hipError_t launch_resolve(const ResolveArgs& a, hipStream_t stream);
 hipError_t launch_finalize(const FinalizeArgs& a, hipStream_t stream);
 hipError_t launch_scatter(const ScatterArgs& a, uint32_t maxN, hipStream_t stream);
+hipError_t launch_inject(const InjectArgs& a, hipStream_t stream);
+hipError_t launch_mt_init_range(uint32_t* st, const uint32_t* seeds, uint32_t Stot, uint32_t off, uint32_t count,
+                                hipStream_t stream);
 
 }  // namespace smd

@@ -182,6 +182,24 @@ smd_status smd_engine_layout(smd_engine* e, uint32_t* total_buckets, uint32_t* k
 smd_status smd_engine_profile_step(smd_engine* e, int nsteps, double* us, int max_stages, char* names, int names_cap,
                                    int* n_stages);
 
+/* Serving (no counterpart in the reference, whose host loop runs one
+ * simulation per process): an engine of `n_slots` empty replica slots, each
+ * owning 2^slot_bucket_bits hash buckets and `slot_streams` RNG streams.
+ * Jobs are admitted into free slots between steps and are stepped together
+ * with every other resident job; a job's results do not depend on what else
+ * is resident.  inject_cap bounds the particles admitted between two steps.
+ * Call on a fresh engine instead of add_replica + finalize. */
+smd_status smd_engine_finalize_slots(smd_engine* e, unsigned n_slots, unsigned slot_bucket_bits,
+                                     unsigned slot_streams, unsigned capacity, unsigned inject_cap);
+/* Load a job into an empty slot: its tables, its streams (seeded from
+ * spec->seed) and its n particles, which enter at the next step's
+ * compaction (the job's first step is the step after that). */
+smd_status smd_engine_admit(smd_engine* e, int slot, const smd_sim_spec* spec, const float* pos4, const int* types,
+                            unsigned n);
+/* Remove every particle of a slot during the next step; the slot is empty
+ * (and may be admitted again) once that step has been enqueued. */
+smd_status smd_engine_retire(smd_engine* e, int slot);
+
 /* Draw `count` numbers from each of `n_streams` fresh streams initialised
  * with (params record k % 32768, seeds[k]) on the GPU: out[k * count + j]. */
 smd_status smd_mt_generate(int device, const uint32_t* seeds, unsigned n_streams, int count, int batch3,
