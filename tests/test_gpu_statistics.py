@@ -1,0 +1,116 @@
+"""Statistical checks of the GPU engine against the oracle and against theory.
+
+  * Two-sample KS test (p > 0.01) of species counts: 256 GPU replicas vs 256
+    oracle replicas with independent seeds and initial positions.
+  * Smoluchowski: A + B -> C measured rate constant equals the requested kf
+    within 2% (the binding radius of rxnparam.cpp:bindingradius is the one
+    whose steady-state rate is kf, reaction_kernel.cuh applies it).
+  * Mass action: A <-> B steady-state fraction equals k2 / (k1 + k2) within
+    2% (first-order probabilities of smolreact.cpp:rxnsetrate).
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+A, B, C = 1, 2, 3
+
+
+def _box_spec(smd, reactions, L, faces, seed, num_streams=64, D=(0.0, 1.0, 1.0, 0.5), grid_bits=(5, 5, 5)):
+    from gpusmoldyn_amd import configs
+
+    return configs.make_spec(D=list(D), dt=1e-3, reactions=reactions, box_lo=[0.0] * 3, box_hi=[L] * 3,
+                             faces=[faces] * 6, grid_bits=grid_bits, num_streams=num_streams, seed=seed)
+
+
+def _uniform(rng, n, L):
+    p = np.empty((n, 4), np.float32)
+    p[:, :3] = rng.uniform(0.0, L, size=(n, 3)).astype(np.float32)
+    p[:, 3] = 1.0
+    return p
+
+
+def test_ks_species_counts_vs_oracle(smd, orc, gpu):
+    from scipy.stats import ks_2samp
+
+    R, nA, nB, L, steps = 256, 300, 300, 10.0, 300
+    rx = [dict(order=2, reactants=[A, B], products=[C], rate=1.0)]
+    types = np.array([A] * nA + [B] * nB, np.int32)
+    e = smd.Engine(0)
+    for k in range(R):
+        spec = _box_spec(smd, rx, L, smd.BC_REFLECT, seed=4357 + k)
+        e.add_replica(spec, _uniform(np.random.default_rng(k), nA + nB, L), types)
+    e.finalize()
+    e.step(steps)
+    gpu_events = (nA + nB) - e.counts().astype(np.int64)  # each A + B -> C removes one particle
+
+    g = smd.gauss_table()
+    cpu_events = np.empty(R, np.int64)
+    for k in range(R):
+        spec = _box_spec(smd, rx, L, smd.BC_REFLECT, seed=91000 + k)
+        sim = orc.sim_for(smd, spec, _uniform(np.random.default_rng(50000 + k), nA + nB, L), types)
+        sim.step(g, steps)
+        cpu_events[k] = (nA + nB) - sim.n
+    res = ks_2samp(gpu_events, cpu_events)
+    print(f"KS: GPU mean {gpu_events.mean():.2f}, oracle mean {cpu_events.mean():.2f}, p = {res.pvalue:.3f}")
+    assert gpu_events.mean() > 5
+    assert res.pvalue > 0.01
+
+
+def smoluchowski_k_hat(counts_fn, step_fn, nA0, n0, V, steps, skip, dt=1e-3):
+    """Maximum-likelihood rate constant of A + B -> C with nA = nB:
+    sum(events) / sum(nA nB dt / V) over steps >= skip.  Each reaction
+    removes one particle, so events and nA follow from the particle counts."""
+    events, exposure = 0, 0.0
+    prev = counts_fn()
+    for t in range(steps):
+        step_fn()
+        nxt = counts_fn()
+        if t >= skip:
+            nA = nA0 - (n0 - prev)
+            events += int((prev - nxt).sum())
+            exposure += float((nA * nA).sum()) * dt / V
+        prev = nxt
+    return events, events / exposure
+
+
+def test_smoluchowski_rate(smd, gpu):
+    """A + B -> C at kf = 0.1 um^3/s (binding radius 0.029 um, a tenth of the
+    diffusion-limited rate, so the Smoluchowski transient is below 1% after
+    20 steps) in a periodic 10 um box: measured rate constant = kf +- 2%."""
+    kf, R, nA0, L, steps, skip = 0.1, 64, 8000, 10.0, 500, 20
+    rx = [dict(order=2, reactants=[A, B], products=[C], rate=kf)]
+    types = np.array([A] * nA0 + [B] * nA0, np.int32)
+    e = smd.Engine(0)
+    for k in range(R):
+        spec = _box_spec(smd, rx, L, smd.BC_PERIODIC, seed=4357 + k, num_streams=512)
+        e.add_replica(spec, _uniform(np.random.default_rng(k), 2 * nA0, L), types)
+    e.finalize()
+    e.set_graphs(False)
+    events, k_hat = smoluchowski_k_hat(lambda: e.counts().astype(np.int64), lambda: e.step(1), nA0, 2 * nA0,
+                                       L ** 3, steps, skip)
+    print(f"Smoluchowski: events {events}, k_hat {k_hat:.5f} vs kf {kf}")
+    assert events > 50000
+    assert abs(k_hat / kf - 1.0) < 0.02
+
+
+def test_mass_action_first_order(smd, gpu):
+    k1, k2, R, n, L = 2.0, 1.0, 64, 2000, 10.0
+    rx = [dict(order=1, reactants=[A], products=[B], rate=k1),
+          dict(order=1, reactants=[B], products=[A], rate=k2)]
+    types = np.full(n, A, np.int32)
+    e = smd.Engine(0)
+    for k in range(R):
+        spec = _box_spec(smd, rx, L, smd.BC_REFLECT, seed=4357 + k, D=(0.0, 1.0, 1.0), num_streams=256)
+        e.add_replica(spec, _uniform(np.random.default_rng(k), n, L), types)
+    e.finalize()
+    e.step(4000)  # 4 s = 12 relaxation times of 1 / (k1 + k2)
+    fa = []
+    for _ in range(20):
+        e.step(200)
+        fa.append(np.mean([e.species_counts(r)[A] for r in range(R)]) / n)
+    frac = float(np.mean(fa))
+    want = k2 / (k1 + k2)
+    print(f"mass action: fraction A {frac:.4f} vs {want:.4f}")
+    assert abs(frac / want - 1.0) < 0.02
