@@ -114,3 +114,31 @@ def test_mass_action_first_order(smd, gpu):
     want = k2 / (k1 + k2)
     print(f"mass action: fraction A {frac:.4f} vs {want:.4f}")
     assert abs(frac / want - 1.0) < 0.02
+
+
+def test_reversible_equilibrium(smd, gpu):
+    """A + B <-> C mass-action steady state within 2%: nC / (nA nB) =
+    k_b / (kr V) with k_b = numrxnrate(step, a, b) / dt, the association rate
+    of rxnparam's reversible radial distribution for the binding radius a and
+    unbinding radius b that smolreact.cpp:393, 459 derive."""
+    import os
+    import sys
+
+    sys.path.insert(0, os.path.dirname(__file__))
+    from test_oracle import reversible_equilibrium, reversible_spec
+
+    kr, want, _ = reversible_equilibrium(smd)
+    R = 256
+    e = smd.Engine(0)
+    types = np.array([A] * 250 + [B] * 250 + [C] * 250, np.int32)
+    for k in range(R):
+        e.add_replica(reversible_spec(kr, 4357 + k), _uniform(np.random.default_rng(k), 750, 10.0), types)
+    e.finalize()
+    e.step(1000)
+    got = []
+    for _ in range(20):
+        e.step(250)
+        got.append(1000 - e.counts().astype(np.int64))  # nA = nB = 500 - nC, so n = 1000 - nC
+    mean = float(np.mean(got))
+    print(f"equilibrium: mean nC {mean:.2f} vs mass action {want:.2f}")
+    assert abs(mean / want - 1.0) < 0.02

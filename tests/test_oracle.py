@@ -228,3 +228,50 @@ def test_smoluchowski_rate(orc, smd):
     k_hat = events / exposure
     assert events > 20000
     assert abs(k_hat / kf - 1.0) < 0.02, k_hat
+
+
+def reversible_spec(kr, seed, kf=0.5, L=10.0, dt=1e-3, num_streams=256):
+    return configs.make_spec(
+        D=[0.0, 1.0, 1.0, 0.5], dt=dt,
+        reactions=[dict(order=2, reactants=[1, 2], products=[3], rate=kf),
+                   dict(order=1, reactants=[3], products=[1, 2], rate=kr, pgem=0.2)],
+        box_lo=[0.0] * 3, box_hi=[L] * 3, faces=[BC_PERIODIC] * 6, grid_bits=(5, 5, 5), num_streams=num_streams,
+        seed=seed, unbind_offsets=1)
+
+
+def reversible_equilibrium(smd, kf=0.5, L=10.0, dt=1e-3, n_total_a=500, target=4e-3):
+    """A + B <-> C with the forward binding radius of the irreversible
+    calibration (smolreact.cpp:393) and products placed at the unbinding
+    radius of pgem = 0.2 (smolreact.cpp:459).  Associations then happen at the
+    rate of rxnparam's reversible radial distribution, k_b =
+    numrxnrate(step, a, b) / dt = kf / (1 - pgem), so the simulated
+    equilibrium is nC / (nA nB) = k_b / (kr V) (not kf / (kr V)).  Returns
+    (kr giving nC / (nA nB) = target, expected mean nC)."""
+    der = smd.derive(reversible_spec(1.0, 1, kf, L, dt))
+    a, b = der[0].bind_radius, der[1].unbind_radius
+    step = math.sqrt(2.0 * 2.0 * dt)
+    kb = smd.numrxnrate(step, a, b) / dt
+    kr = kb / (L ** 3 * target)
+    # nC = target (n - nC)^2
+    q = target
+    nC = ((2 * n_total_a * q + 1) - math.sqrt((2 * n_total_a * q + 1) ** 2 - 4 * q * q * n_total_a ** 2)) / (2 * q)
+    return kr, nC, kb
+
+
+def test_reversible_equilibrium(orc, smd):
+    """Mass-action steady state of A + B <-> C within 2% (16 replicas,
+    sampled every 250 steps after 1000 steps from a near-equilibrium start)."""
+    kr, want, kb = reversible_equilibrium(smd)
+    assert abs(kb / 0.5 - 1.25) < 0.01  # kf / (1 - pgem)
+    g = smd.gauss_table()
+    got = []
+    for k in range(16):
+        pos = configs.uniform_box(np.random.default_rng(k), 750, 0.0, 10.0)
+        types = np.array([1] * 250 + [2] * 250 + [3] * 250, np.int32)
+        sim = orc.sim_for(smd, reversible_spec(kr, 4357 + k), pos, types)
+        sim.step(g, 1000)
+        for _ in range(20):
+            sim.step(g, 250)
+            got.append(1000 - sim.n)  # nA = nB = 500 - nC, so n = 1000 - nC
+    mean = float(np.mean(got))
+    assert abs(mean / want - 1.0) < 0.02, (mean, want)
