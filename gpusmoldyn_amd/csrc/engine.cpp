@@ -68,7 +68,14 @@ Engine::~Engine() {
   dfree(dFirst);
   dfree(dRetire);
   dfree(dInjCount);
-  dfree(dInjUse);
+  dfree(dInjBase);
+  dfree(dStatus);
+  dfree(dZMt);
+  dfree(dZPar);
+  dfree(dZCount);
+  dfree(dZeroBase);
+  dfree(dZPos);
+  dfree(dZType);
   dfree(dInjSrcOff);
   dfree(dInjPos);
   dfree(dInjType);
@@ -161,6 +168,8 @@ smd_status Engine::finalize(uint32_t capacity) {
   uint64_t buckets = 0, streams = 0, total = 0;
   anySplit = false;
   anyBim = false;
+  anyZero = false;
+  uint64_t zeros = 0;
   for (uint32_t r = 0; r < R; ++r) {
     ReplicaTables& T = tabs[r];
     T.bucketOff = (uint32_t)buckets;
@@ -171,19 +180,22 @@ smd_status Engine::finalize(uint32_t capacity) {
     for (int x = 0; x < T.numRxn; ++x)
       if (specs[r].rxn[x].order == 1 && T.prodY[x] > 0) anySplit = true;
     if (T.hasBim) anyBim = true;
+    T.zeroOff = (uint32_t)zeros;
+    zeros += T.zeroMax;
+    if (T.numZero) anyZero = true;
   }
-  if (buckets >= (1ull << 31) || streams >= (1ull << 31)) {
-    set_error("too many buckets or streams");
+  if (buckets >= (1ull << 31) || streams >= (1ull << 31) || zeros >= (1ull << 30)) {
+    set_error("too many buckets, streams or order-0 creations");
     return SMD_ERR_INVALID;
   }
+  zeroTotal = (uint32_t)zeros;
   deadKey = (uint32_t)buckets;
   keyBits = 0;
   while ((1ull << keyBits) <= deadKey) ++keyBits;
   nb = deadKey + 1;
   Stot = (uint32_t)streams;
   uint64_t c = capacity ? capacity : std::max<uint64_t>(anySplit ? 8 * total : total, 1024);
-  if (c < total) c = total;
-  if (anySplit && c < 2 * total) c = 2 * total;
+  if (c < step_bound(total)) c = step_bound(total);
   if (c >= (1ull << 31)) {
     set_error("capacity too large");
     return SMD_ERR_CAPACITY;
@@ -264,6 +276,38 @@ smd_status Engine::finalize(uint32_t capacity) {
   HIPCHK(launch_mt_init(dMtState, dSeeds, Stot, stream));
   HIPCHK(hipStreamSynchronize(stream));
   dfree(dSeeds);
+  HIPCHK(dalloc(&dStatus, 1));
+  HIPCHK(hipMemset(dStatus, 0, sizeof(uint32_t)));
+  sb.status = dStatus;
+  if (anyZero) {
+    // Replica r's creation stream: the next stream after its S particle
+    // streams (seed S of its seed sequence, MersenneTwister.dat record S).
+    std::vector<uint32_t> zs(R), zp(3 * (size_t)R);
+    for (uint32_t r = 0; r < R; ++r) {
+      const uint32_t S = tabs[r].numStreams;
+      std::vector<uint32_t> sq(S + 1);
+      if (!stream_seeds(specs[r].seed, S + 1, sq.data(), err)) {
+        set_error(err);
+        return SMD_ERR_IO;
+      }
+      zs[r] = sq[S];
+      const uint32_t rec = S % kMtParamSets;
+      for (int w = 0; w < 3; ++w) zp[(size_t)w * R + r] = dat[4 * rec + w];
+    }
+    HIPCHK(dalloc(&dZMt, (size_t)kMtWords * R));
+    HIPCHK(dalloc(&dZPar, 3 * (size_t)R));
+    HIPCHK(dalloc(&dZCount, R));
+    HIPCHK(dalloc(&dZeroBase, R));
+    HIPCHK(dalloc(&dZPos, zeroTotal));
+    HIPCHK(dalloc(&dZType, zeroTotal));
+    HIPCHK(hipMemcpy(dZPar, zp.data(), zp.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    uint32_t* dzs = nullptr;
+    HIPCHK(dalloc(&dzs, R));
+    HIPCHK(hipMemcpy(dzs, zs.data(), R * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIPCHK(launch_mt_init(dZMt, dzs, R, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    dfree(dzs);
+  }
 
   // Particles.
   smd_status st = alloc_particle_buffers((uint32_t)c);
@@ -297,8 +341,7 @@ void Engine::refresh_bound(bool block) {
     knownN = pinned[pendingHead];
     const uint64_t lag = stepsDone - p.step;
     uint64_t b = knownN;
-    if (anySplit)
-      for (uint64_t k = 0; k < lag && b < (1ull << 40); ++k) b *= 2;
+    for (uint64_t k = 0; k < lag && b < (1ull << 40); ++k) b = step_bound(b);
     Nub = std::min<uint64_t>(Nub, b);
     pendingHead = (pendingHead + 1) % kRing;
     --pendingCount;
@@ -338,10 +381,10 @@ smd_status Engine::grow(uint64_t need) {
 
 smd_status Engine::ensure_capacity() {
   refresh_bound(false);
-  const uint64_t needAfter = anySplit ? 2 * Nub : Nub;
+  const uint64_t needAfter = step_bound(Nub);
   if (needAfter <= cap) return SMD_OK;
   refresh_bound(true);
-  const uint64_t need2 = anySplit ? 2 * Nub : Nub;
+  const uint64_t need2 = step_bound(Nub);
   if (need2 <= cap) return SMD_OK;
   return grow(2 * need2);
 }
@@ -351,7 +394,7 @@ smd_status Engine::enqueue_step(StageTimer* timer) {
   smd_status st = enqueue_kernels(maxN, timer);
   if (st != SMD_OK) return st;
   ++stepsDone;
-  if (anySplit) Nub = std::min<uint64_t>(2 * Nub, (uint64_t)cap);
+  Nub = std::min<uint64_t>(step_bound(Nub), (uint64_t)cap);
   return enqueue_readback();
 }
 
@@ -377,17 +420,17 @@ void Engine::drop_graph() {
 smd_status Engine::step_graph(int pairs) {
   for (int k = 0; k < pairs; ++k) {
     refresh_bound(false);
-    uint64_t need = anySplit ? 4 * Nub : Nub;  // output of the second step
+    uint64_t need = step_bound(step_bound(Nub));  // output of the second step
     if (need > cap) {
       refresh_bound(true);
-      need = anySplit ? 4 * Nub : Nub;
+      need = step_bound(step_bound(Nub));
       if (need > cap) {
         drop_graph();
         smd_status st = grow(2 * need);
         if (st != SMD_OK) return st;
       }
     }
-    const uint64_t bound = anySplit ? 2 * Nub : Nub;  // input bound of the second step
+    const uint64_t bound = step_bound(Nub);  // input bound of the second step
     if (gexec && (cur != gParity || bound > gMaxN)) drop_graph();
     if (!gexec) {
       const uint32_t maxN = (uint32_t)std::min<uint64_t>(bound, cap);
@@ -406,7 +449,7 @@ smd_status Engine::step_graph(int pairs) {
     }
     HIPCHK(hipGraphLaunch(gexec, stream));
     stepsDone += 2;
-    if (anySplit) Nub = std::min<uint64_t>(4 * Nub, (uint64_t)cap);
+    Nub = std::min<uint64_t>(step_bound(step_bound(Nub)), (uint64_t)cap);
     smd_status st = enqueue_readback();
     if (st != SMD_OK) return st;
   }
@@ -439,6 +482,18 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
   pa.tileBirth = dTileBirth;
   pa.retire = dRetire;
   HIPCHK(launch_producer(pa, stream));
+  if (anyZero) {
+    ZeroArgs za;
+    za.tables = dTables;
+    za.R = (int)R;
+    za.zMt = dZMt;
+    za.zPar = dZPar;
+    za.zPos = dZPos;
+    za.zType = dZType;
+    za.zCount = dZCount;
+    za.status = dStatus;
+    HIPCHK(launch_zero(za, stream));
+  }
   mark(1);
   if (anyBim) {
     HIPCHK(launch_sort(sb, dN, maxN, keyBits, radixBits, true, stream));
@@ -495,7 +550,9 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
   fa.tileBirthPre = dTileBirthPre;
   fa.particleSteps = dParticleSteps;
   fa.injCount = dInjCount;
-  fa.injUse = dInjUse;
+  fa.zCount = anyZero ? dZCount : nullptr;
+  fa.zeroBase = dZeroBase;
+  fa.injBase = dInjBase;
   fa.retire = dRetire;
   HIPCHK(launch_finalize(fa, stream));
   mark(7);
@@ -516,6 +573,9 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
   sa.posOut = dPos[cur ^ 1];
   sa.typeOut = dType[cur ^ 1];
   HIPCHK(launch_scatter(sa, maxN, stream));
+  if (anyZero)
+    HIPCHK(launch_zero_place((int)R, dTables, dZPos, dZType, dZCount, dNewOff, dZeroBase, dPos[cur ^ 1],
+                             dType[cur ^ 1], stream));
   if (injN) {
     InjectArgs ia;
     ia.injPos = dInjPos;
@@ -524,9 +584,7 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
     ia.injSrcOff = dInjSrcOff;
     ia.nInj = injN;
     ia.newOff = dNewOff;
-    ia.newCount = dNewCount;
-    ia.repAlive = dRepAlive;
-    ia.injUse = dInjUse;
+    ia.injBase = dInjBase;
     ia.posOut = dPos[cur ^ 1];
     ia.typeOut = dType[cur ^ 1];
     HIPCHK(launch_inject(ia, stream));
@@ -576,6 +634,23 @@ smd_status Engine::sync() {
   HIPCHK(hipSetDevice(device));
   HIPCHK(hipStreamSynchronize(stream));
   refresh_bound(true);
+  return check_status();
+}
+
+// Error bits the kernels set since finalize (they stay set: the state after
+// such a step is not trustworthy).
+smd_status Engine::check_status() {
+  if (!dStatus) return SMD_OK;
+  uint32_t v = 0;
+  HIPCHK(hipMemcpy(&v, dStatus, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (v & kStatusZeroOverflow) {
+    set_error("order-0 reactions created more particles in one step than the per-step limit");
+    return SMD_ERR_CAPACITY;
+  }
+  if (v & kStatusSortSpin) {
+    set_error("sort look-back wait exceeded its bound");
+    return SMD_ERR_HIP;
+  }
   return SMD_OK;
 }
 
@@ -621,7 +696,7 @@ smd_status Engine::finalize_slots(uint32_t nSlots, uint32_t slotBucketBits, uint
   HIPCHK(hipMemcpy(dStreamReplica, none.data(), Stot * sizeof(int), hipMemcpyHostToDevice));
   for (uint32_t r = 0; r < R; ++r) tabs[r].numStreams = 0;
   HIPCHK(hipMemcpy(dTables, tabs.data(), R * sizeof(ReplicaTables), hipMemcpyHostToDevice));
-  for (uint32_t** p : {&dRetire, &dInjCount, &dInjUse, &dInjSrcOff}) {
+  for (uint32_t** p : {&dRetire, &dInjCount, &dInjBase, &dInjSrcOff}) {
     HIPCHK(dalloc(p, R));
     HIPCHK(hipMemset(*p, 0, R * sizeof(uint32_t)));
   }
@@ -651,6 +726,10 @@ smd_status Engine::admit(int slot, const smd_sim_spec& spec, const float* pos4, 
   if (!derive_tables(spec, T, der.data(), err)) {
     set_error(err);
     return SMD_ERR_RXN_SETUP;
+  }
+  if (T.numZero) {
+    set_error("order-0 reactions are not supported in slot engines");
+    return SMD_ERR_INVALID;
   }
   if (T.gbits[0] + T.gbits[1] + T.gbits[2] > slotBits || T.numStreams > slotStreams) {
     set_error("job grid or stream count exceeds the slot");
@@ -734,7 +813,7 @@ smd_status Engine::set_particles(const float* pos4, const int* types, uint32_t n
     }
   smd_status st = sync();
   if (st != SMD_OK) return st;
-  if ((uint64_t)n * (anySplit ? 2 : 1) > cap) {
+  if (step_bound(n) > cap) {
     knownN = 0;
     st = grow(4ull * n);
     if (st != SMD_OK) return st;

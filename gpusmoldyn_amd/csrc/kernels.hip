@@ -523,7 +523,7 @@ __global__ __launch_bounds__(256) void k_os_pass(const uint32_t* __restrict__ ke
                 __hip_atomic_load(status + (size_t)tp * RADIX + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if ((v & (kOsPrefix | kOsAgg)) == 0u) {
               if (++spins > kOsSpinLimit) {
-                atomicOr(err, 1u);
+                atomicOr(err, kStatusSortSpin);
                 break;
               }
               __builtin_amdgcn_s_sleep(1);
@@ -952,13 +952,15 @@ __global__ __launch_bounds__(1024) void k_finalize(FinalizeArgs a) {
   uint32_t cN = 0, cA = 0, cB = 0;
   for (uint32_t b = 0; b < (uint32_t)a.R; b += 1024) {
     const uint32_t r = b + threadIdx.x;
-    uint32_t alive = 0, births = 0, inj = 0;
+    uint32_t alive = 0, births = 0, zc = 0, inj = 0;
     if (r < (uint32_t)a.R) {
       alive = a.repCount[r] - a.repDeaths[r];
       births = a.repBirths[r];
+      if (a.zCount) zc = a.zCount[r];
       if (a.injCount) inj = a.injCount[r];
     }
-    const uint32_t exN = block_exscan1024(alive + births + inj, wsum, &tot);
+    // new segment of replica r: [survivors, births, created, admitted]
+    const uint32_t exN = block_exscan1024(alive + births + zc + inj, wsum, &tot);
     const uint32_t tN = tot;
     __syncthreads();
     const uint32_t exA = block_exscan1024(alive, wsum, &tot);
@@ -968,14 +970,15 @@ __global__ __launch_bounds__(1024) void k_finalize(FinalizeArgs a) {
     const uint32_t tB = tot;
     if (r < (uint32_t)a.R) {
       a.newOff[r] = cN + exN;
-      a.newCount[r] = alive + births + inj;
+      a.newCount[r] = alive + births + zc + inj;
       a.aliveBase[r] = cA + exA;
       a.birthBase[r] = cB + exB;
       a.repAlive[r] = alive;
       a.repDeaths[r] = 0;
       a.repBirths[r] = 0;
+      if (a.zCount) a.zeroBase[r] = alive + births;
       if (a.injCount) {
-        a.injUse[r] = inj;
+        a.injBase[r] = alive + births + zc;
         a.injCount[r] = 0;
       }
       if (a.retire) a.retire[r] = 0;
@@ -1091,10 +1094,79 @@ __global__ __launch_bounds__(256) void k_inject(InjectArgs a) {
   const uint32_t q = blockIdx.x * 256 + threadIdx.x;
   if (q >= a.nInj) return;
   const int r = a.injRep[q];
-  const uint32_t births = a.newCount[r] - a.repAlive[r] - a.injUse[r];
-  const uint32_t dst = a.newOff[r] + a.repAlive[r] + births + (q - a.injSrcOff[r]);
+  const uint32_t dst = a.newOff[r] + a.injBase[r] + (q - a.injSrcOff[r]);
   a.posOut[dst] = a.injPos[q];
   a.typeOut[dst] = a.injType[q];
+}
+
+// ---------------------------------------------------------------------------
+// Order-0 reactions (reaction_cpu.cpp:36-63, run on the host by the reference
+// before each step and appended after the births, particleSystem.cpp:833-837,
+// 989-991).  One thread per replica draws, from the replica's own creation
+// stream, for each order-0 reaction in index order: the Poisson count
+// (stats.cpp:poisrandF multiplication method: multiply closed-interval
+// uniforms until the product drops to exp(-mean)), then for each new
+// particle x, y, z = lo + u (hi - lo) with u on the open interval
+// (particleSystem.h:randPos, stats.h:unirandOOF), w = 1.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float real_closed(uint32_t v) { return (float)((double)v * (1.0 / 4294967295.0)); }
+__device__ __forceinline__ float real_open(uint32_t v) { return (float)(((double)v + 0.5) * (1.0 / 4294967296.0)); }
+
+__global__ __launch_bounds__(64) void k_zero(ZeroArgs a) {
+  __shared__ uint32_t lds_mt[kMtNN * 64];
+  const int r = blockIdx.x * 64 + threadIdx.x;
+  if (r >= a.R) return;
+  const ReplicaTables& T = a.tables[r];
+  if (T.numZero == 0) {
+    a.zCount[r] = 0u;
+    return;
+  }
+  Mt m;
+  mt_load<64>(m, &lds_mt[threadIdx.x], a.zMt, a.zPar, (uint32_t)a.R, (uint32_t)r);
+  const uint32_t cap = T.zeroMax;
+  uint32_t cnt = 0;
+  bool over = false;
+  for (int q = 0; q < T.numZero && !over; ++q) {
+    const int rx = T.zeroList[q];
+    const float g = T.zeroG[q];
+    uint32_t nmol = 0;
+    for (int p = 0; p < T.zeroPieces[q] && nmol <= cap; ++p) {
+      // bounded: a count above cap is an error anyway
+      for (float t = real_closed(mt_draw<64>(m)); t > g && nmol <= cap; t *= real_closed(mt_draw<64>(m))) ++nmol;
+    }
+    const int type = T.prodX[rx];
+    for (uint32_t j = 0; j < nmol; ++j) {
+      if (cnt >= cap) {
+        over = true;
+        break;
+      }
+      float x[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) x[d] = real_open(mt_draw<64>(m)) * T.L[d] + T.lo[d];
+      a.zPos[T.zeroOff + cnt] = make_float4(x[0], x[1], x[2], 1.0f);
+      a.zType[T.zeroOff + cnt] = type;
+      ++cnt;
+    }
+  }
+  if (over) atomicOr(a.status, kStatusZeroOverflow);
+  a.zCount[r] = cnt;
+  mt_store<64>(m, a.zMt, (uint32_t)a.R, (uint32_t)r);
+}
+
+__global__ __launch_bounds__(64) void k_zero_place(const ReplicaTables* __restrict__ tables,
+                                                   const float4* __restrict__ zPos, const int* __restrict__ zType,
+                                                   const uint32_t* __restrict__ zCount,
+                                                   const uint32_t* __restrict__ newOff,
+                                                   const uint32_t* __restrict__ zeroBase, float4* __restrict__ posOut,
+                                                   int* __restrict__ typeOut) {
+  const int r = blockIdx.x;
+  const uint32_t c = zCount[r];
+  if (c == 0) return;
+  const uint32_t src = tables[r].zeroOff, dst = newOff[r] + zeroBase[r];
+  for (uint32_t j = threadIdx.x; j < c; j += 64) {
+    posOut[dst + j] = zPos[src + j];
+    typeOut[dst + j] = zType[src + j];
+  }
 }
 
 __global__ __launch_bounds__(256) void k_mt_init_range(uint32_t* __restrict__ st, const uint32_t* __restrict__ seeds,
@@ -1156,7 +1228,7 @@ static hipError_t launch_sort_onesweep(SortBuffers& b, const uint32_t* dN, uint3
   const int passes = sort_passes(keyBits, radixBits);
   uint32_t* hist = b.counts;
   uint32_t* counters = hist + (size_t)passes * radix;
-  uint32_t* err = counters + passes;
+  uint32_t* err = b.status ? b.status : counters + passes;
   uint32_t* status = err + 4;
   const uint32_t zeroWords = (uint32_t)((size_t)passes * radix + passes + 4 + (size_t)passes * maxTiles * radix);
   hipLaunchKernelGGL(k_os_zero, dim3(std::min<unsigned>(cdiv(zeroWords, 256), 1024)), dim3(256), 0, stream, b.counts,
@@ -1256,6 +1328,21 @@ hipError_t launch_finalize(const FinalizeArgs& a, hipStream_t stream) {
 hipError_t launch_inject(const InjectArgs& a, hipStream_t stream) {
   if (a.nInj == 0) return hipSuccess;
   hipLaunchKernelGGL(k_inject, dim3(cdiv(a.nInj, 256)), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero(const ZeroArgs& a, hipStream_t stream) {
+  if (a.R <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero, dim3(cdiv(a.R, 64)), dim3(64), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_place(int R, const ReplicaTables* tables, const float4* zPos, const int* zType,
+                             const uint32_t* zCount, const uint32_t* newOff, const uint32_t* zeroBase, float4* posOut,
+                             int* typeOut, hipStream_t stream) {
+  if (R <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero_place, dim3(R), dim3(64), 0, stream, tables, zPos, zType, zCount, newOff, zeroBase,
+                     posOut, typeOut);
   return hipGetLastError();
 }
 

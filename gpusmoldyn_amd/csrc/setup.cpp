@@ -507,7 +507,11 @@ bool derive_tables(const smd_sim_spec& sp, ReplicaTables& T, smd_rxn_derived* de
     const smd_reaction& R = sp.rxn[r];
     if (R.rate < 0) { code[r] = 1; continue; }
     if (R.order == 0) {
-      prob[r] = R.rate * sp.dt * sp.volume;
+      // getSystemVolume(): the spec's volume, or the box volume when unset
+      const float vol = sp.volume > 0 ? sp.volume
+                                      : (sp.box_hi[0] - sp.box_lo[0]) * (sp.box_hi[1] - sp.box_lo[1]) *
+                                            (sp.box_hi[2] - sp.box_lo[2]);
+      prob[r] = R.rate * sp.dt * vol;
     } else if (R.order == 1) {
       const int i = R.reactants[0];
       const OrderCsr& c = csr[1];
@@ -614,6 +618,25 @@ bool derive_tables(const smd_sim_spec& sp, ReplicaTables& T, smd_rxn_derived* de
       err = "reaction " + std::to_string(r) + ": setup error code " + std::to_string(code[r]);
       return false;
     }
+
+  // Order-0 creation, in reaction index order (reaction_cpu.cpp:zeroReact).
+  // poisrandF (stats.cpp:234-275) uses the multiplication method below a mean
+  // of 12; larger means are split into equal pieces below 12 (a sum of
+  // independent Poisson variates is Poisson with the summed mean).
+  double zeroMean = 0.0;
+  for (int r = 0; r < NR; ++r) {
+    const smd_reaction& R = sp.rxn[r];
+    if (R.order != 0 || R.nprod < 1 || !(prob[r] > 0.0f)) continue;
+    const int k = T.numZero++;
+    const int pieces = prob[r] < 12.0f ? 1 : (int)std::ceil(prob[r] / 8.0f);
+    const float xm = pieces == 1 ? prob[r] : prob[r] / (float)pieces;
+    T.zeroList[k] = r;
+    T.zeroPieces[k] = pieces;
+    T.zeroG[k] = (float)std::exp(-(double)xm);
+    zeroMean += prob[r];
+  }
+  // Creations beyond mean + 10 sd (+32) in one step are reported as an error.
+  T.zeroMax = T.numZero ? (uint32_t)std::ceil(zeroMean + 10.0 * std::sqrt(zeroMean) + 32.0) : 0u;
 
   // Pair-search filters derived from the final pair table.
   for (int a = 0; a < NT; ++a) {

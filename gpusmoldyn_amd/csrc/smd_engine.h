@@ -45,7 +45,8 @@ class Engine {
   std::vector<std::vector<float>> pos0;
   std::vector<std::vector<int>> type0;
   uint32_t R = 0, Stot = 0, deadKey = 0, keyBits = 0, nb = 0, cap = 0, nTilesCap = 0;
-  bool anySplit = false, anyBim = false;
+  bool anySplit = false, anyBim = false, anyZero = false;
+  uint32_t zeroTotal = 0;  // order-0 creations per step, all replicas, at most
   uint64_t Nub = 0, knownN = 0, stepsDone = 0;
   int cur = 0;
 
@@ -70,6 +71,11 @@ class Engine {
   int* dCand = nullptr;
   SortBuffers sb = {nullptr, nullptr, nullptr, nullptr, nullptr};
   uint32_t *dTileAlive = nullptr, *dTileBirth = nullptr, *dTileAlivePre = nullptr, *dTileBirthPre = nullptr;
+  uint32_t* dStatus = nullptr;  // sticky kernel error bits
+  // order-0 creation: one stream per replica, a per-step creation buffer
+  uint32_t *dZMt = nullptr, *dZPar = nullptr, *dZCount = nullptr, *dZeroBase = nullptr;
+  float4* dZPos = nullptr;
+  int* dZType = nullptr;
 
   // asynchronous count readback
   struct Pending {
@@ -88,7 +94,7 @@ class Engine {
   smd_status retire(int slot);
   bool slotsMode = false;
   uint32_t slotBits = 0, slotStreams = 0;
-  uint32_t *dRetire = nullptr, *dInjCount = nullptr, *dInjUse = nullptr, *dInjSrcOff = nullptr;
+  uint32_t *dRetire = nullptr, *dInjCount = nullptr, *dInjBase = nullptr, *dInjSrcOff = nullptr;
   float4* dInjPos = nullptr;
   int* dInjType = nullptr;
   int* dInjRep = nullptr;
@@ -111,6 +117,9 @@ class Engine {
   void refresh_bound(bool block);
   smd_status grow(uint64_t need);
   smd_status ensure_capacity();
+  // Upper bound of the particle count after one step that starts with x.
+  uint64_t step_bound(uint64_t x) const { return (anySplit ? 2 * x : x) + zeroTotal; }
+  smd_status check_status();
   smd_status enqueue_step(StageTimer* timer);
 };
 

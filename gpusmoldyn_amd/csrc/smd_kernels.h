@@ -35,6 +35,7 @@ struct SortBuffers {
   uint32_t* keysAlt;
   uint32_t* valsAlt;
   uint32_t* counts;  // sort_counts_words(maxN, radixBits)
+  uint32_t* status;  // sticky status word (kStatusSortSpin), or nullptr
 };
 
 struct CandArgs {
@@ -81,9 +82,22 @@ struct FinalizeArgs {
   uint32_t* tileAlivePre;
   uint32_t* tileBirthPre;
   unsigned long long* particleSteps;  // += particles entering the step
-  uint32_t* injCount;  // per replica: particles admitted this step (nullptr: none); reset
-  uint32_t* injUse;    // per replica: copy of injCount for the injection kernel
-  uint32_t* retire;    // per replica retire flags (nullptr: none); reset
+  const uint32_t* zCount;  // per replica: particles created this step (nullptr: none)
+  uint32_t* zeroBase;      // per replica: alive + births, where created particles go
+  uint32_t* injCount;      // per replica: particles admitted this step (nullptr: none); reset
+  uint32_t* injBase;       // per replica: alive + births + created, where admitted particles go
+  uint32_t* retire;        // per replica retire flags (nullptr: none); reset
+};
+
+struct ZeroArgs {
+  const ReplicaTables* tables;
+  int R;
+  uint32_t* zMt;         // [20][R] one creation stream per replica
+  const uint32_t* zPar;  // [3][R]
+  float4* zPos;          // creation buffer (replica r at tables[r].zeroOff)
+  int* zType;
+  uint32_t* zCount;      // per replica: particles created this step
+  uint32_t* status;      // sticky error bits (kStatusZeroOverflow)
 };
 
 struct InjectArgs {
@@ -93,9 +107,7 @@ struct InjectArgs {
   const uint32_t* injSrcOff;  // first admitted particle of each replica
   uint32_t nInj;
   const uint32_t* newOff;
-  const uint32_t* newCount;
-  const uint32_t* repAlive;
-  const uint32_t* injUse;
+  const uint32_t* injBase;
   float4* posOut;
   int* typeOut;
 };
@@ -135,6 +147,11 @@ hipError_t launch_resolve(const ResolveArgs& a, hipStream_t stream);
 hipError_t launch_finalize(const FinalizeArgs& a, hipStream_t stream);
 hipError_t launch_scatter(const ScatterArgs& a, uint32_t maxN, hipStream_t stream);
 hipError_t launch_inject(const InjectArgs& a, hipStream_t stream);
+hipError_t launch_zero(const ZeroArgs& a, hipStream_t stream);
+// Appends replica r's created particles after its survivors and births.
+hipError_t launch_zero_place(int R, const ReplicaTables* tables, const float4* zPos, const int* zType,
+                             const uint32_t* zCount, const uint32_t* newOff, const uint32_t* zeroBase, float4* posOut,
+                             int* typeOut, hipStream_t stream);
 hipError_t launch_mt_init_range(uint32_t* st, const uint32_t* seeds, uint32_t Stot, uint32_t off, uint32_t count,
                                 hipStream_t stream);
 

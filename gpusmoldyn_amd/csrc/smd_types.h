@@ -25,6 +25,10 @@ enum Boundary : int32_t {
   kPeriodic = 3,  // wrap (both faces of the axis must be periodic)
 };
 
+// Sticky engine status bits, set by kernels and reported by the host.
+constexpr uint32_t kStatusSortSpin = 1u;      // a sort look-back wait hit its bound
+constexpr uint32_t kStatusZeroOverflow = 2u;  // order-0 creations exceeded the per-step limit
+
 struct ReplicaTables {
   int32_t numTypes;
   int32_t numRxn;
@@ -67,7 +71,17 @@ struct ReplicaTables {
   // random streams
   uint32_t streamOff;      // first global stream (multiple of 64)
   uint32_t numStreams;     // S: particle i of this replica draws from stream i % S
-  uint32_t pad_[2];
+  // order-0 reactions (smolreact.cpp:295-303, reaction_cpu.cpp:36-63): per
+  // step, Poisson(prob) new particles of type prodX at uniform positions in
+  // the box.  Poisson(prob) is drawn as the sum of zeroPieces Poisson
+  // variates of mean prob / zeroPieces (each below 12), each by the
+  // multiplication method of stats.cpp:poisrandF against zeroG = exp(-mean).
+  int32_t numZero;
+  int32_t zeroList[kMaxRxn];
+  int32_t zeroPieces[kMaxRxn];
+  float zeroG[kMaxRxn];
+  uint32_t zeroMax;        // creations per step above this are an error
+  uint32_t zeroOff;        // first slot of this replica in the creation buffer
 };
 
 }  // namespace smd

@@ -4,6 +4,7 @@ Used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the
 checker.  The product package never imports this module.
 """
 import ctypes
+import math
 import os
 import subprocess
 
@@ -32,6 +33,8 @@ class OrcNet(ctypes.Structure):
         ("unbind_offsets", ctypes.c_int),
         ("origin", ctypes.c_float * 3), ("cell", ctypes.c_float * 3),
         ("gbits", ctypes.c_uint * 3), ("num_streams", ctypes.c_uint),
+        ("num_zero", ctypes.c_int), ("zero_list", ctypes.c_int * MAXR), ("zero_pieces", ctypes.c_int * MAXR),
+        ("zero_g", ctypes.c_float * MAXR),
     ]
 
 
@@ -40,6 +43,7 @@ class OrcSim(ctypes.Structure):
         ("net", OrcNet), ("n", ctypes.c_uint), ("cap", ctypes.c_uint),
         ("pos", ctypes.c_void_p), ("type", ctypes.c_void_p),
         ("mt", ctypes.c_void_p), ("par", ctypes.c_void_p),
+        ("zmt", ctypes.c_void_p), ("zpar", ctypes.c_void_p),
     ]
 
 
@@ -124,6 +128,29 @@ def net_from_spec(spec, derived, cell, origin):
             n.sphere[s][q] = spec.sphere[s][q]
     n.unbind_offsets = spec.unbind_offsets
     n.num_streams = spec.num_streams or 32768
+    # order-0: mean = k dt V per step (smolreact.cpp:295-303), V the spec's
+    # volume or the box volume; Poisson by the multiplication method
+    # (stats.cpp:234-253), means of 12 and above split into pieces below 12
+    vol = np.float32(spec.volume)
+    if not vol > 0:
+        ext = [np.float32(spec.box_hi[d]) - np.float32(spec.box_lo[d]) for d in range(3)]
+        vol = ext[0] * ext[1] * ext[2]
+    k = 0
+    for r in range(R):
+        x = spec.rxn[r]
+        if x.order != 0 or x.nprod < 1:
+            continue
+        mean = np.float32(x.rate) * dt * vol
+        if not mean > 0:
+            continue
+        pieces = 1 if mean < np.float32(12.0) else int(np.ceil(mean / np.float32(8.0)))
+        xm = mean if pieces == 1 else mean / np.float32(pieces)
+        n.zero_list[k] = r
+        n.zero_pieces[k] = pieces
+        n.zero_g[k] = float(np.float32(math.exp(-float(xm))))
+        n.prob[r] = float(mean)
+        k += 1
+    n.num_zero = k
     return n
 
 
@@ -148,6 +175,14 @@ class Sim:
         self.s.n, self.s.cap = n, cap
         self.s.pos, self.s.type = _p(self.pos), _p(self.type)
         self.s.mt, self.s.par = _p(self.mt), _p(self.par)
+        self.zmt = np.zeros(20, np.uint32)
+        self.zpar = np.zeros(3, np.uint32)
+        self.s.zmt, self.s.zpar = _p(self.zmt), _p(self.zpar)
+
+    def set_creation_stream(self, seed, params):
+        """Order-0 creation stream: seed and 3 parameter words."""
+        lib().orc_mt_init(_p(self.zmt), int(seed))
+        self.zpar[:] = np.asarray(params, np.uint32)[:3]
 
     @property
     def n(self):
@@ -189,6 +224,10 @@ def sim_for(smd, spec, pos, types, cap=None):
     cell, origin = smd.spec_cells(spec)
     net = net_from_spec(spec, der, cell, origin)
     S = net.num_streams
-    seeds = smd.stream_seeds(spec.seed, S)
+    seeds = smd.stream_seeds(spec.seed, S + 1)
     dat = load_dat(os.path.join(os.path.dirname(smd.__file__), "data", "MersenneTwister.dat"))
-    return Sim(net, pos, types, seeds, stream_params(S, dat), cap=cap)
+    sim = Sim(net, pos, types, seeds[:S], stream_params(S, dat), cap=cap)
+    if net.num_zero:
+        # the stream after the S particle streams: seed S, record S mod 32768
+        sim.set_creation_stream(seeds[S], dat[S % 32768, :3])
+    return sim

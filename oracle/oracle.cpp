@@ -456,6 +456,32 @@ int compact(orc_sim& s, StepScratch& w) {
   return 0;
 }
 
+// Order-0 creation after the compaction: for each order-0 reaction in index
+// order, a Poisson count (product of closed-interval uniforms compared with
+// exp(-mean)), then per new particle x, y, z = lo + u (hi - lo) with u on the
+// open interval, w = 1.
+int create_zeroth(orc_sim& s) {
+  const orc_net& n = s.net;
+  if (n.num_zero == 0) return 0;
+  Stream z{s.zmt, s.zpar};
+  auto closed = [&z]() { return (float)((double)z.draw() * (1.0 / 4294967295.0)); };
+  auto open = [&z]() { return (float)(((double)z.draw() + 0.5) * (1.0 / 4294967296.0)); };
+  for (int q = 0; q < n.num_zero; ++q) {
+    const int rx = n.zero_list[q];
+    unsigned nmol = 0;
+    for (int p = 0; p < n.zero_pieces[q]; ++p)
+      for (float t = closed(); t > n.zero_g[q]; t *= closed()) ++nmol;
+    for (unsigned j = 0; j < nmol; ++j) {
+      if (s.n >= s.cap) return 5;
+      float* p = &s.pos[4 * (size_t)s.n];
+      for (int d = 0; d < 3; ++d) p[d] = open() * (n.hi[d] - n.lo[d]) + n.lo[d];
+      p[3] = 1.0f;
+      s.type[s.n++] = n.prod_x[rx];
+    }
+  }
+  return 0;
+}
+
 uint32_t dead_key(const orc_net& n) { return 1u << (n.gbits[0] + n.gbits[1] + n.gbits[2]); }
 
 }  // namespace
@@ -472,7 +498,9 @@ int orc_step(orc_sim* s, const float* gauss, int nsteps) {
       candidates(*s, w);
       resolve(*s, w);
     }
-    const int rc = compact(*s, w);
+    int rc = compact(*s, w);
+    if (rc) return rc;
+    rc = create_zeroth(*s);
     if (rc) return rc;
   }
   return 0;

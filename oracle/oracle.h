@@ -18,6 +18,8 @@
  *   reaction_kernel.cuh:263-342    uniReact
  *   particles_kernel.cu:97-123     compactPosAndTypeD (order-preserving intent)
  *   particleSystem.cpp:829-1001    update(): stage order, births appended after survivors
+ *   reaction_cpu.cpp:36-63         zeroReact (order-0 creation, appended after the births)
+ *   stats.cpp:234-275              poisrandF (multiplication method)
  *   radixsort.cu (semantics only)  stable sort of (hash, index) by hash
  *   surfaces_kernel.cuh:97-248     sphere sidedness / crossing semantics
  *
@@ -68,6 +70,13 @@ typedef struct orc_net {
   float origin[3], cell[3];
   unsigned gbits[3];
   unsigned num_streams;
+  /* order-0 reactions in index order: Poisson(mean) creations per step,
+   * mean = prob[zero_list[k]], drawn as zero_pieces[k] Poisson variates by
+   * the multiplication method against zero_g[k] = exp(-mean / pieces) */
+  int num_zero;
+  int zero_list[ORC_MAX_RXN];
+  int zero_pieces[ORC_MAX_RXN];
+  float zero_g[ORC_MAX_RXN];
 } orc_net;
 
 /* One simulation.  pos: cap*4 floats, type: cap ints, mt: num_streams*20
@@ -79,6 +88,8 @@ typedef struct orc_sim {
   int* type;
   uint32_t* mt;
   const uint32_t* par;
+  uint32_t* zmt;        /* creation stream (20 words), used when num_zero > 0 */
+  const uint32_t* zpar; /* its 3 parameter words */
 } orc_sim;
 
 /* Stream init (MersenneTwisterInitialise) and draws. */

@@ -113,8 +113,28 @@ def w_clamp():
     return spec, pos, types
 
 
+def w_zeroth(seed=4357):
+    # order-0 creation (one mean below 12, one split into pieces) feeding
+    # A + B -> C and C -> 0
+    V = 125.0
+    spec = configs.make_spec(
+        D=[0.0, 1.0, 1.0, 0.5], dt=1e-2,
+        reactions=[dict(order=0, products=[1], rate=4.0 / (1e-2 * V)),
+                   dict(order=2, reactants=[1, 2], products=[3], rate=2.0),
+                   dict(order=0, products=[2], rate=40.0 / (1e-2 * V)),
+                   dict(order=1, reactants=[3], products=[], rate=2.0)],
+        box_lo=[0.0] * 3, box_hi=[5.0] * 3, faces=[BC_REFLECT] * 6, grid_bits=(4, 4, 4), num_streams=300,
+        seed=seed, volume=V)
+    rng = np.random.default_rng(14)
+    n = 3000
+    pos = configs.uniform_box(rng, n, 0.0, 5.0)
+    types = rng.integers(1, 4, size=n).astype(np.int32)
+    return spec, pos, types
+
+
 WORKLOADS = {
     "c1_small": w_c1_small,
+    "zeroth": w_zeroth,
     "periodic_reversible": w_periodic_reversible,
     "offsets": w_offsets,
     "c3_small": w_c3_small,
@@ -152,7 +172,7 @@ def test_stage_parity(smd, orc, gpu, name):
             assert np.array_equal(got["cand"], ref["cand"]), "pair candidates"
 
 
-@pytest.mark.parametrize("name,steps", [("c1_small", 60), ("periodic_reversible", 40), ("offsets", 40),
+@pytest.mark.parametrize("name,steps", [("c1_small", 60), ("zeroth", 40), ("periodic_reversible", 40), ("offsets", 40),
                                         ("c3_small", 20), ("clamp_reference_test3", 30)])
 def test_multistep_parity(smd, orc, gpu, name, steps):
     spec, pos, types = WORKLOADS[name]()
@@ -192,8 +212,8 @@ def test_c1_full_run_matches_oracle(smd, orc, gpu):
 
 
 def test_ensemble_equals_alone(smd, gpu):
-    reps = [w_c1_small(), w_periodic_reversible(), w_offsets(), w_clamp()]
-    banner("ensemble of 4 vs alone")
+    reps = [w_c1_small(), w_zeroth(11), w_periodic_reversible(), w_offsets(), w_zeroth(12), w_clamp()]
+    banner("ensemble of 6 vs alone")
     e = smd.Engine(0)
     for spec, pos, types in reps:
         e.add_replica(spec, pos, types)
@@ -277,7 +297,7 @@ def test_extinction_and_empty(smd, gpu):
 def test_graph_replay_equals_eager(smd, gpu):
     """Pairs of steps replayed from a captured hipGraph give the eager result,
     replay after replay, with eager steps, syncs and host reads in between."""
-    for name in ("periodic_reversible", "c1_small"):
+    for name in ("periodic_reversible", "c1_small", "zeroth"):
         spec, pos, types = WORKLOADS[name]()
         banner(f"graph vs eager {name}")
         a = _engine(smd, spec, pos, types)

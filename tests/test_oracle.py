@@ -275,3 +275,66 @@ def test_reversible_equilibrium(orc, smd):
             got.append(1000 - sim.n)  # nA = nB = 500 - nC, so n = 1000 - nC
     mean = float(np.mean(got))
     assert abs(mean / want - 1.0) < 0.02, (mean, want)
+
+
+def _creation_spec(means, k_decay=0.0, L=10.0, dt=1e-2, seed=4357):
+    V = L ** 3
+    rx = [dict(order=0, products=[1], rate=m / (dt * V)) for m in means]
+    if k_decay:
+        rx.append(dict(order=1, reactants=[1], products=[], rate=k_decay))
+    return configs.make_spec(D=[0.0, 0.0], dt=dt, reactions=rx, box_lo=[0.0] * 3, box_hi=[L] * 3,
+                             faces=[BC_REFLECT] * 6, grid_bits=(3, 3, 3), num_streams=64, seed=seed, volume=V)
+
+
+@pytest.mark.parametrize("mean", [3.0, 30.0])
+def test_zeroth_order_counts_are_poisson(orc, smd, mean):
+    """Creations per step follow Poisson(k dt V): the multiplication method
+    below a mean of 12 and the sum of pieces above it (chi-square over the
+    count histogram, p > 0.001), at uniform positions inside the box."""
+    from scipy import stats
+
+    spec = _creation_spec([mean])
+    der = smd.derive(spec)
+    assert der[0].prob == np.float32(np.float32(spec.rxn[0].rate) * np.float32(spec.dt) * np.float32(spec.volume))
+    steps = 3000
+    sim = orc.sim_for(smd, spec, np.zeros((0, 4), np.float32), np.zeros(0, np.int32), cap=int(mean * steps * 2) + 64)
+    g = smd.gauss_table()
+    counts = []
+    for _ in range(steps):
+        n0 = sim.n
+        sim.step(g, 1)
+        counts.append(sim.n - n0)
+    counts = np.array(counts)
+    lo, hi = int(stats.poisson.ppf(0.001, mean)), int(stats.poisson.ppf(0.999, mean))
+    edges = np.arange(lo, hi + 2)
+    obs = np.array([(counts < lo).sum()] + [(counts == k).sum() for k in edges[:-1]] + [(counts > hi).sum()])
+    p = np.concatenate([[stats.poisson.cdf(lo - 1, mean)], stats.poisson.pmf(edges[:-1], mean),
+                        [stats.poisson.sf(hi, mean)]])
+    keep = p * steps >= 5
+    chi = stats.chisquare(obs[keep], p[keep] / p[keep].sum() * obs[keep].sum())
+    assert chi.pvalue > 1e-3, (counts.mean(), counts.var(), chi)
+    pos, t = sim.state()
+    assert np.all(t == 1)
+    # u = (v + 1/2) / 2^32 on the open interval, rounded to float: u = 1.0f
+    # (x = hi) for the top 128 outputs, never u = 0
+    assert np.all((pos[:, :3] > 0) & (pos[:, :3] <= 10.0)) and np.all(pos[:, 3] == 1.0)
+    assert np.all(np.abs(pos[:, :3].mean(axis=0) - 5.0) < 0.1)
+
+
+def test_zeroth_order_steady_state(orc, smd):
+    """0 -> A (k0) with A -> 0 (k1): mass-action steady state. Decay acts
+    before the step's creations, so n = (n + Poisson(k0 V dt)) thinned by
+    exp(-k1 dt) gives mean k0 V dt / (1 - exp(-k1 dt)); within 2%."""
+    lam, k1, dt = 5.0, 2.0, 1e-2
+    spec = _creation_spec([lam], k_decay=k1, dt=dt)
+    want = lam / (1.0 - math.exp(-k1 * dt))
+    g = smd.gauss_table()
+    got = []
+    for rep in range(4):
+        spec.seed = 4357 + rep
+        sim = orc.sim_for(smd, spec, np.zeros((0, 4), np.float32), np.zeros(0, np.int32), cap=1 << 14)
+        sim.step(g, 400)  # 8 relaxation times
+        for _ in range(40):
+            sim.step(g, 25)
+            got.append(sim.n)
+    assert abs(np.mean(got) / want - 1.0) < 0.02, (np.mean(got), want)
