@@ -181,6 +181,8 @@ def main():
     ap.add_argument("--no-graphs", action="store_true", help="one launch sequence per step instead of graphs")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--jobs", type=int, default=200, help="c5: number of jobs")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="also time a plain copy kernel over BYTES (reference for hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); replicas and slabs would shard "
@@ -189,6 +191,10 @@ def main():
     if args.config == "c5":
         print(json.dumps(serve(args)), flush=True)
         return
+    probe = None
+    if args.copy_probe:
+        us = smd.copy_probe(args.copy_probe, reps=10)
+        probe = {"bytes": args.copy_probe, "avg_us": us, "gbs": 2 * args.copy_probe / (us * 1e-6) / 1e9}
     reps, steps, desc = build(args.config)
     steps = args.steps if args.steps is not None else steps
     e = make_engine(reps, args.radix_bits)
@@ -214,6 +220,8 @@ def main():
         "roofline": rl,
         "step_roofline": step,
     }
+    if probe:
+        out["copy_probe"] = probe
     if args.config == "c4" and not args.no_plain:
         # the same 512-replica workload, one simulation per launch sequence
         pps, psecs = 0, 0.0

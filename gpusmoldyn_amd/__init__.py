@@ -112,6 +112,7 @@ def lib():
         L.smd_mt_generate.argtypes = [ctypes.c_int, u32p, ctypes.c_uint, ctypes.c_int, ctypes.c_int, u32p]
         L.smd_sort_pairs.argtypes = [ctypes.c_int, u32p, u32p, ctypes.c_uint, ctypes.c_uint, ctypes.c_int]
         L.smd_gauss_table.argtypes = [f32p, ctypes.c_int]
+        L.smd_copy_probe.argtypes = [ctypes.c_int, ctypes.c_ulonglong, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         for f, n in (("numrxnrate", 3), ("actrxnrate", 2), ("unbindingradius", 4)):
             fn = getattr(L, "smd_" + f)
             fn.restype = ctypes.c_double
@@ -209,6 +210,13 @@ def sort_pairs(keys, values, key_bits, radix_bits=8, device=0):
     v = np.ascontiguousarray(values, np.uint32).copy()
     _check(lib().smd_sort_pairs(device, _ptr(k), _ptr(v), len(k), key_bits, radix_bits), "smd_sort_pairs")
     return k, v
+
+
+def copy_probe(nbytes, reps=10, device=0):
+    """Average duration (us) of a plain float4 copy kernel over nbytes."""
+    us = ctypes.c_double(0)
+    _check(lib().smd_copy_probe(device, nbytes, reps, ctypes.byref(us)), "smd_copy_probe")
+    return us.value
 
 
 # ---------------------------------------------------------------- engine

@@ -210,6 +210,12 @@ smd_status smd_mt_generate(int device, const uint32_t* seeds, unsigned n_streams
 smd_status smd_sort_pairs(int device, uint32_t* keys, uint32_t* values, unsigned n, unsigned key_bits,
                           int radix_bits);
 
+/* Measurement helper (no counterpart in the reference): times `reps`
+ * launches of a plain float4 copy kernel over `bytes` bytes (read once,
+ * written once) on `device`; *us = average launch duration.  Used to scale
+ * hardware byte counters (FETCH_SIZE, WRITE_SIZE) to bytes moved. */
+smd_status smd_copy_probe(int device, unsigned long long bytes, int reps, double* us);
+
 /* ---- ParticleSystem (particleSystem.h:50-218) through C ----
  * The C++ class is declared in particle_system.h; these wrappers expose it to
  * C and foreign-function callers.  smd_ps_update returns 1 (true), 0 (false:
