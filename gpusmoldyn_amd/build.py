@@ -50,7 +50,25 @@ def build(verbose=False):
     if verbose and out.strip():
         print(out)
     os.replace(tmp, LIB)
+    build_driver(verbose)
     return LIB
+
+
+DRIVER = os.path.join(HERE, "smd_particles")
+
+
+def build_driver(verbose=False):
+    """The headless particles.cpp-style driver, linked against the library."""
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", inc, os.path.join(CSRC, "particles_main.cpp"),
+           "-L", HERE, "-lgpusmoldyn", "-Wl,-rpath,$ORIGIN", "-o", DRIVER + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("driver build failed:\n" + res.stdout + res.stderr)
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

@@ -2,6 +2,7 @@
 way ParticleSystem::update() drives it (particleSystem.cpp:829-1001), and the
 ParticleSystem class itself, checked bit for bit against the engine."""
 import ctypes
+import os
 
 import numpy as np
 import pytest
@@ -310,3 +311,39 @@ def test_particle_system_update_false_on_extinction(smd, gpu):
     res = [L.smd_ps_update(ps) for _ in range(10)]
     assert set(res) <= {0, 1} and res[-1] == 0  # false once no particle is left
     L.smd_ps_destroy(ps)
+
+
+def test_headless_driver_qatest(smd, gpu, tmp_path):
+    """smd_particles -qatest (particles.cpp benchmark/QA mode without GL):
+    the dumped positions equal ParticleSystem(n, 64^3) + CONFIG_GRID reset +
+    the same number of update() calls, and a second run compares PASSED
+    against the first dump."""
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(smd.__file__), "smd_particles")
+    assert os.path.exists(exe), "driver not built"
+    n, it = 8192, 12
+    out = tmp_path / "particles.bin"
+    print("\n[gpu-test] headless driver", flush=True)
+    r = subprocess.run([exe, f"-n={n}", f"-i={it}", "-qatest", f"-out={out}", f"-ref={tmp_path / 'none.bin'}"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert f"{n} particles, total time for {it} iterations" in r.stdout
+    got = np.fromfile(out, np.float32).reshape(-1, 4)
+    L = smd.lib()
+    L.smd_ps_create_reference.restype = vp
+    L.smd_ps_create_reference.argtypes = [ctypes.c_uint] * 4
+    for f in ("smd_ps_update", "smd_ps_reset", "smd_ps_destroy"):
+        getattr(L, f).argtypes = [vp] + ([ctypes.c_int] if f == "smd_ps_reset" else [])
+    L.smd_ps_get_positions.argtypes = [vp, vp, ctypes.c_uint]
+    ps = L.smd_ps_create_reference(n, 64, 64, 64)
+    assert L.smd_ps_reset(ps, 1) == 0
+    for _ in range(it):
+        assert L.smd_ps_update(ps) == 1
+    want = np.zeros((n, 4), np.float32)
+    L.smd_ps_get_positions(ps, want.ctypes.data_as(vp), n)
+    L.smd_ps_destroy(ps)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    r2 = subprocess.run([exe, f"-n={n}", f"-i={it}", "-qatest", f"-out={tmp_path / 'again.bin'}", f"-ref={out}"],
+                        capture_output=True, text=True, timeout=120)
+    assert r2.returncode == 0 and "TEST PASSED!" in r2.stdout, r2.stdout + r2.stderr
