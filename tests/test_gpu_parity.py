@@ -313,3 +313,20 @@ def test_graph_replay_equals_eager(smd, gpu):
         assert a.particle_steps() == b.particle_steps()
         a.close()
         b.close()
+
+
+def test_autotune_picks_a_valid_fastest_config(smd, gpu):
+    from gpusmoldyn_amd import autotune
+
+    spec, pos, types = w_c1_small()
+    banner("autotune")
+    best, table = autotune.tune(spec, pos, types, steps=5, max_delta=1)
+    assert best in table and table[best] == min(table.values())
+    assert len(table) >= 2
+    # the tuned grid still runs and conserves A + C and B + C
+    e = smd.Engine(0, best[1])
+    e.add_replica(autotune.apply(spec, best[0]), pos, types)
+    e.finalize()
+    e.step(20)
+    c = e.species_counts(0)
+    assert c[1] + c[3] == (types == 1).sum() and c[2] + c[3] == (types == 2).sum()
