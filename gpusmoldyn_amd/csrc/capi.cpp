@@ -67,6 +67,31 @@ double smd_unbindingradius(double pgem, double dt, double difc, double a) {
   return smd::unbindingradius(pgem, dt, difc, a);
 }
 
+double smd_rdf_steady_state(const double* r, double* rdfa, double* rdfd, int n, double step, double b, double eps) {
+  if (!r || !rdfa || !rdfd || n < 10 || !(step > 0) || !(eps > 0)) return -1.0;
+  return smd::rdf_steady_state(r, rdfa, rdfd, n, step, b, eps);
+}
+
+smd_status smd_rdf_sweep(char mode, int n, double eps, int down, double b, double* flux31) {
+  if (!flux31 || n < 10 || !(eps > 0) || (mode != 'i' && mode != 'r' && mode != 'b') ||
+      (mode == 'r' && !(b > 1.0)) || (mode == 'b' && !(b >= 0.0 && b <= 1.0))) {
+    set_error("invalid rdf sweep arguments");
+    return SMD_ERR_INVALID;
+  }
+  return smd::rdf_sweep(mode, n, eps, down != 0, b, flux31) == 0 ? SMD_OK : SMD_ERR_INVALID;
+}
+
+smd_status smd_rdf_grid(double rmax, int n, double* r) {
+  if (!r || n < 10 || !(rmax > 1.1)) {
+    set_error("invalid rdf grid arguments");
+    return SMD_ERR_INVALID;
+  }
+  std::vector<double> tmp(n + 2);
+  smd::rdf_grid(rmax, n, tmp.data());
+  std::memcpy(r, tmp.data(), n * sizeof(double));
+  return SMD_OK;
+}
+
 smd_status smd_spec_derive(const smd_sim_spec* spec, smd_rxn_derived* out) {
   if (!spec) {
     set_error("null spec");

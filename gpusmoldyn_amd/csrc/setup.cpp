@@ -253,6 +253,9 @@ void step_nodes(int sidx, double x[4]) {
   for (int i = 0; i < 4; i++) x[i] = kSLo + (sidx - 1 + i) * kSInc;
 }
 
+
+}  // namespace
+
 double erfc_d(double x) {
   // Numerical Recipes erfcc (rxnparam.cpp:316-322)
   const double z = std::fabs(x);
@@ -267,8 +270,6 @@ double erfc_d(double x) {
                                             t * (-1.13520398 + t * (1.48851587 + t * (-0.82215223 + t * 0.17087277)))))))));
   return x >= 0.0 ? ans : 2.0 - ans;
 }
-
-}  // namespace
 
 double numrxnrate(double step, double a, double b) {
   if (step < 0 || a < 0) return -1;

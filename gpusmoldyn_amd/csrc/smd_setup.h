@@ -14,6 +14,14 @@ double numrxnrate(double step, double a, double b);
 double actrxnrate(double step, double a);
 double bindingradius(double rate, double dt, double difc, double b, int rel);
 double unbindingradius(double pgem, double dt, double difc, double a);
+double erfc_d(double x);  // rxnparam_erfccD (rxnparam.cpp:316-322)
+// rdf tables (rxnparam.cpp:325-564), reduced units (binding radius 1)
+double rdf_absorb(const double* r, double* rdf, int n);
+void rdf_diffuse(const double* r, const double* rdfa, double* rdfd, int n, double step);
+void rdf_reverse(const double* r, double* rdf, int n, double step, double b, double flux);
+double rdf_steady_state(const double* r, double* rdfa, double* rdfd, int n, double step, double b, double eps);
+void rdf_grid(double rmax, int n, double* r);
+int rdf_sweep(char mode, int n, double eps, bool down, double b, double* flux31);
 
 // Builds the device tables of one replica from its spec (fillReactionOrderInfo,
 // rxnsetrate, rxnsetproduct, molSetTimestep, grid and geometry checks).

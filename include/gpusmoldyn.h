@@ -121,6 +121,23 @@ double smd_numrxnrate(double step, double a, double b);
 double smd_actrxnrate(double step, double a);
 double smd_bindingradius(double rate, double dt, double difc, double b, int rel);
 double smd_unbindingradius(double pgem, double dt, double difc, double a);
+/* Radial distribution function of a reactive pair and the rate tables of
+ * rxnparam (rxnparam.cpp:325-564), FP64, lengths reduced by the binding
+ * radius.  smd_rdf_steady_state iterates diffusion (rms step `step`), the
+ * reverse-reaction source at b (b >= 0; b < 0: irreversible) and absorption
+ * inside r = 1 to a steady flux (relative change < eps); rdfa is the trial rdf
+ * in and the absorbed rdf out, rdfd the diffused rdf out; returns the flux per
+ * step (reduced rate), or -1.  smd_rdf_sweep reproduces one column of the
+ * table maker (rdfmaketable, rxnparam.cpp:477-564): mode 'i' (irreversible),
+ * 'r' (b > 1) or 'b' (0 <= b <= 1), n radial points, the 31 reduced steps
+ * exp(-3) .. exp(3) swept upwards (down = 0) or downwards, each steady state
+ * starting from the previous; flux31[k] belongs to step exp(-3) exp(0.2)^k. */
+double smd_rdf_steady_state(const double* r, double* rdfa, double* rdfd, int n, double step, double b, double eps);
+smd_status smd_rdf_sweep(char mode, int n, double eps, int down, double b, double* flux31);
+/* The radial grid of the table maker: 0, rmax/n, ..., with the points next
+ * to 1 moved to 0.9999 and 1.0001 (n points). */
+smd_status smd_rdf_grid(double rmax, int n, double* r);
+
 /* Derived reaction parameters of a spec (smolreact.cpp:173-483). */
 smd_status smd_spec_derive(const smd_sim_spec* spec, smd_rxn_derived* out);
 /* Hash-grid cell size and origin the engine uses for a spec (periodic axes

@@ -9,6 +9,7 @@
 //   numrxnrate / actrxnrate      rxnparam.cpp:42-239
 //   bindingradius                rxnparam.cpp:253-277
 //   unbindingradius              rxnparam.cpp:287-308
+//   rdfsteadystate               rxnparam.cpp:451-475
 #include <cstddef>
 #include <cstdint>
 
@@ -55,6 +56,10 @@ double smdref_bindingradius(double rate, double dt, double difc, double b, int r
 }
 double smdref_unbindingradius(double pgem, double dt, double difc, double a) {
   return unbindingradius(pgem, dt, difc, a);
+}
+
+double smdref_rdf_steady_state(double* r, double* rdfa, double* rdfd, int n, double step, double b, double eps) {
+  return rdfsteadystate(r, rdfa, rdfd, n, step, b, eps);
 }
 
 }  // extern "C"

@@ -7,6 +7,8 @@ from rnd/genmtrand.cpp, stats.cpp, SFMT/SFMT.cpp and rxnparam.cpp).
   gauss_table.npy  createGaussTable(4096)
   rxnparam.npz     numrxnrate / actrxnrate / bindingradius / unbindingradius
                    over a parameter grid that includes every workload value
+  rdf.npz          rdfsteadystate (irreversible and reversible, b < 1 and
+                   b > 1) on small radial grids: flux and both output rdfs
 
 Run here (the reference tree is not present on GPU machines):
     make -C oracle ref && python tests/golden/make_fixtures.py
@@ -80,7 +82,48 @@ def main():
     np.savez_compressed(os.path.join(HERE, "rxnparam.npz"), numrxnrate=np.array(nr), actrxnrate=np.array(ar),
                         bindingradius=np.array(br), bindingradius_rel=np.array(brr),
                         unbindingradius=np.array(ub))
+    write_rdf(L)
     print("fixtures written to", HERE)
+
+
+def rdf_grid(rmax, n):
+    # the table maker's grid (rxnparam.cpp:500-505)
+    r = np.zeros(n)
+    dr = rmax / n
+    i = 1
+    while r[i - 1] < 1 and i - 1 < n:
+        r[i] = r[i - 1] + dr
+        i += 1
+    r[i - 1] = 0.9999
+    r[i] = 1.0001
+    i += 1
+    while i < n:
+        r[i] = r[i - 1] + dr
+        i += 1
+    return r
+
+
+def write_rdf(L):
+    dbl = ctypes.c_double
+    L.smdref_rdf_steady_state.restype = dbl
+    L.smdref_rdf_steady_state.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int, dbl, dbl, dbl]
+    cases = [(10.0, 60, 0.3, -1.0), (10.0, 60, 2.0, -1.0), (5.0, 50, 0.7, 0.5), (5.0, 50, 1.5, 0.0),
+             (5.0, 50, 1.0, 2.0)]
+    rows = []
+    for rmax, n, step, b in cases:
+        r = rdf_grid(rmax, n)
+        a0 = np.where(r < 1, 0.0, 1.0 - 1.0 / np.maximum(r, 1e-300))
+        a = a0.copy()
+        d = np.zeros(n)
+        flux = L.smdref_rdf_steady_state(r.ctypes.data_as(ctypes.c_void_p), a.ctypes.data_as(ctypes.c_void_p),
+                                         d.ctypes.data_as(ctypes.c_void_p), n, step, b, 1e-6)
+        rows.append((rmax, n, step, b, flux, r, a0, a, d))
+    np.savez_compressed(os.path.join(HERE, "rdf.npz"),
+                        params=np.array([(rm, n, s, b, f) for rm, n, s, b, f, *_ in rows]),
+                        **{f"r{k}": row[5] for k, row in enumerate(rows)},
+                        **{f"init{k}": row[6] for k, row in enumerate(rows)},
+                        **{f"absorbed{k}": row[7] for k, row in enumerate(rows)},
+                        **{f"diffused{k}": row[8] for k, row in enumerate(rows)})
 
 
 if __name__ == "__main__":
