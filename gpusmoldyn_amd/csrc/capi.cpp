@@ -74,7 +74,7 @@ double smd_rdf_steady_state(const double* r, double* rdfa, double* rdfd, int n, 
 
 smd_status smd_rdf_sweep(char mode, int n, double eps, int down, double b, double* flux31) {
   if (!flux31 || n < 10 || !(eps > 0) || (mode != 'i' && mode != 'r' && mode != 'b') ||
-      (mode == 'r' && !(b > 1.0)) || (mode == 'b' && !(b >= 0.0 && b <= 1.0))) {
+      (mode == 'r' && !(b >= 1.0)) || (mode == 'b' && !(b >= 0.0 && b <= 1.0))) {
     set_error("invalid rdf sweep arguments");
     return SMD_ERR_INVALID;
   }

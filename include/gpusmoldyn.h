@@ -129,7 +129,7 @@ double smd_unbindingradius(double pgem, double dt, double difc, double a);
  * in and the absorbed rdf out, rdfd the diffused rdf out; returns the flux per
  * step (reduced rate), or -1.  smd_rdf_sweep reproduces one column of the
  * table maker (rdfmaketable, rxnparam.cpp:477-564): mode 'i' (irreversible),
- * 'r' (b > 1) or 'b' (0 <= b <= 1), n radial points, the 31 reduced steps
+ * 'r' (b >= 1) or 'b' (0 <= b <= 1), n radial points, the 31 reduced steps
  * exp(-3) .. exp(3) swept upwards (down = 0) or downwards, each steady state
  * starting from the previous; flux31[k] belongs to step exp(-3) exp(0.2)^k. */
 double smd_rdf_steady_state(const double* r, double* rdfa, double* rdfd, int n, double step, double b, double eps);
@@ -137,6 +137,11 @@ smd_status smd_rdf_sweep(char mode, int n, double eps, int down, double b, doubl
 /* The radial grid of the table maker: 0, rmax/n, ..., with the points next
  * to 1 moved to 0.9999 and 1.0001 (n points). */
 smd_status smd_rdf_grid(double rmax, int n, double* r);
+/* Many sweeps at once on GPU `device` (one workgroup per sweep, n <= 1024):
+ * sweep q has mode modes[q], unbinding radius bs[q] and direction downs[q];
+ * flux[31 q + k] belongs to reduced step exp(-3) exp(0.2)^k. */
+smd_status smd_rdf_sweeps_gpu(int device, int n, double eps, int n_sweeps, const char* modes, const double* bs,
+                              const int* downs, double* flux);
 
 /* Derived reaction parameters of a spec (smolreact.cpp:173-483). */
 smd_status smd_spec_derive(const smd_sim_spec* spec, smd_rxn_derived* out);
