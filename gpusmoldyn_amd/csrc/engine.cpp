@@ -590,6 +590,8 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
     HIPCHK(launch_inject(ia, stream));
     injN = 0;
   }
+  // every admission (empty ones too) has entered with this step
+  std::fill(slotPending.begin(), slotPending.end(), (uint8_t)0);
   mark(8);
   cur ^= 1;
   std::swap(dRepOff, dNewOff);
@@ -704,6 +706,7 @@ smd_status Engine::finalize_slots(uint32_t nSlots, uint32_t slotBucketBits, uint
   HIPCHK(dalloc(&dInjType, injCap));
   HIPCHK(dalloc(&dInjRep, injCap));
   slotBusy.assign(R, 0);
+  slotPending.assign(R, 0);
   return SMD_OK;
 }
 
@@ -712,8 +715,8 @@ smd_status Engine::admit(int slot, const smd_sim_spec& spec, const float* pos4, 
     set_error("admit needs a slot engine and a valid slot");
     return SMD_ERR_INVALID;
   }
-  if (slotBusy[slot]) {
-    set_error("slot is occupied (retire its job first)");
+  if (slotBusy[slot] || slotPending[slot]) {
+    set_error("slot is occupied (retire its job first, and step once after an admission)");
     return SMD_ERR_INVALID;
   }
   if (injN + n > injCap) {
@@ -785,12 +788,17 @@ smd_status Engine::admit(int slot, const smd_sim_spec& spec, const float* pos4, 
   injN += n;
   Nub += n;
   slotBusy[slot] = 1;
+  slotPending[slot] = 1;
   return SMD_OK;
 }
 
 smd_status Engine::retire(int slot) {
   if (!slotsMode || slot < 0 || (uint32_t)slot >= R) {
     set_error("retire needs a slot engine and a valid slot");
+    return SMD_ERR_INVALID;
+  }
+  if (slotPending[slot]) {
+    set_error("the slot's job has not entered yet (step once after an admission)");
     return SMD_ERR_INVALID;
   }
   smd_status st = sync();

@@ -1228,8 +1228,11 @@ static hipError_t launch_sort_onesweep(SortBuffers& b, const uint32_t* dN, uint3
   const int passes = sort_passes(keyBits, radixBits);
   uint32_t* hist = b.counts;
   uint32_t* counters = hist + (size_t)passes * radix;
+  // layout inside b.counts: hist | tile counters | 4 words | per-pass status
+  // words; a look-back overrun is flagged in the caller's sticky status word
+  // when there is one, else in the first of the 4 words
+  uint32_t* status = counters + passes + 4;
   uint32_t* err = b.status ? b.status : counters + passes;
-  uint32_t* status = err + 4;
   const uint32_t zeroWords = (uint32_t)((size_t)passes * radix + passes + 4 + (size_t)passes * maxTiles * radix);
   hipLaunchKernelGGL(k_os_zero, dim3(std::min<unsigned>(cdiv(zeroWords, 256), 1024)), dim3(256), 0, stream, b.counts,
                      zeroWords);
@@ -1246,10 +1249,10 @@ static hipError_t launch_sort_onesweep(SortBuffers& b, const uint32_t* dN, uint3
     uint32_t* st = status + (size_t)p * maxTiles * radix;
     if (radixBits == 8)
       hipLaunchKernelGGL(k_os_pass<8>, dim3(maxTiles), dim3(256), 0, stream, b.keys, b.vals, b.keysAlt, b.valsAlt, dN,
-                         maxTiles * (uint32_t)kSortTile, hist + (size_t)p * radix, st, counters + p, err, shift, idv);
+                         maxN, hist + (size_t)p * radix, st, counters + p, err, shift, idv);
     else
       hipLaunchKernelGGL(k_os_pass<4>, dim3(maxTiles), dim3(256), 0, stream, b.keys, b.vals, b.keysAlt, b.valsAlt, dN,
-                         maxTiles * (uint32_t)kSortTile, hist + (size_t)p * radix, st, counters + p, err, shift, idv);
+                         maxN, hist + (size_t)p * radix, st, counters + p, err, shift, idv);
     std::swap(b.keys, b.keysAlt);
     std::swap(b.vals, b.valsAlt);
   }

@@ -99,7 +99,8 @@ class Engine {
   int* dInjType = nullptr;
   int* dInjRep = nullptr;
   uint32_t injCap = 0, injN = 0;
-  std::vector<uint8_t> slotBusy;  // admitted and not yet retired
+  std::vector<uint8_t> slotBusy;     // admitted and not yet retired
+  std::vector<uint8_t> slotPending;  // admitted, particles not yet entered
 
   // hipGraph replay of two-step pairs
   bool useGraphs = true;
