@@ -883,9 +883,9 @@ __global__ __launch_bounds__(kBS) void k_resolve(ResolveArgs a) {
     const float dc1 = T.D[t1], dc2 = T.D[t2];
     const float x = (dc1 == 0.0f && dc2 == 0.0f) ? 0.5f : dc2 / (dc1 + dc2);
     // pos2 taken at its image nearest pos1 on periodic axes
-    const float q0 = p1.x - min_image(p1.x - p2.x, T, 0);
-    const float q1 = p1.y - min_image(p1.y - p2.y, T, 1);
-    const float q2 = p1.z - min_image(p1.z - p2.z, T, 2);
+    const float q0 = nearest_image(p1.x, p2.x, T, 0);
+    const float q1 = nearest_image(p1.y, p2.y, T, 1);
+    const float q2 = nearest_image(p1.z, p2.z, T, 2);
     const float omx = 1.0f - x;
     float rp[3] = {fmaf(x, p1.x, omx * q0), fmaf(x, p1.y, omx * q1), fmaf(x, p1.z, omx * q2)};
     const float th = smd_theta(rand_real(mt_draw<kBS>(m)));

@@ -183,6 +183,18 @@ __device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, fl
 }
 
 // Minimum-image component along a periodic axis.
+// The image of p2 nearest p1: p2 itself unless the axis is periodic and the
+// pair is more than half a box apart (then p2 shifted by one box length), so
+// that non-wrapping pairs use p2 exactly as bimReact does.
+__device__ __forceinline__ float nearest_image(float p1, float p2, const ReplicaTables& T, int axis) {
+  if (T.periodic[axis]) {
+    const float d = p1 - p2;
+    if (d > T.halfL[axis]) return p2 + T.L[axis];
+    if (d < -T.halfL[axis]) return p2 - T.L[axis];
+  }
+  return p2;
+}
+
 __device__ __forceinline__ float min_image(float d, const ReplicaTables& T, int axis) {
   if (T.periodic[axis]) {
     if (d > T.halfL[axis]) d -= T.L[axis];

@@ -131,6 +131,19 @@ void rotated_offsets(float theta, float phi, float chi, const float* pp, float v
 
 inline bool periodic_axis(const orc_net& n, int d) { return n.face[2 * d] == 3; }
 
+// The image of p2 nearest p1: p2 itself unless the axis is periodic and the
+// pair is more than half a box apart (then p2 shifted by one box length), so
+// that non-wrapping pairs use p2 exactly as bimReact does.
+inline float nearest_image(const orc_net& n, int d, float p1, float p2) {
+  if (!periodic_axis(n, d)) return p2;
+  const float L = n.hi[d] - n.lo[d];
+  const float h = 0.5f * L;
+  const float v = p1 - p2;
+  if (v > h) return p2 + L;
+  if (v < -h) return p2 - L;
+  return p2;
+}
+
 inline float wrap_delta(const orc_net& n, int d, float v) {
   if (!periodic_axis(n, d)) return v;
   const float L = n.hi[d] - n.lo[d];
@@ -398,7 +411,7 @@ void resolve(orc_sim& s, StepScratch& w) {
     const float omx = 1.0f - x;
     float at[3];
     for (int d = 0; d < 3; ++d) {
-      const float img = P1[d] - wrap_delta(n, d, P1[d] - P2[d]);
+      const float img = nearest_image(n, d, P1[d], P2[d]);
       at[d] = std::fma(x, P1[d], omx * img);
     }
     const float th = orc_theta(uniform01(rng.draw()));

@@ -35,7 +35,7 @@ def legacy_spec(n_types_D, reactions, grid_bits=6):
                              origin=[-1.0] * 3, num_streams=32768, particle_radius=1.0 / 64.0)
 
 
-def test1_like():
+def network_test1_like():
     # particleSystem.cpp:319-406 (A -> 2A, A + B -> 2B, B -> 0) with rates that
     # keep the population bounded over the test
     reactions = [dict(order=1, reactants=[1], products=[1, 1], rate=0.4),
@@ -213,7 +213,7 @@ def _lib(smd):
 
 def test_legacy_update_loop_matches_engine(smd, gpu):
     L = _lib(smd)
-    spec, pos, types = test1_like()
+    spec, pos, types = network_test1_like()
     print("\n[gpu-test] legacy update loop, test1-like network, 20000 particles", flush=True)
     drv = LegacyDriver(L, smd, spec, pos, types, cap=1 << 18)
     e = smd.Engine(0)
