@@ -278,7 +278,16 @@ def test_particle_system_reference_ctor(smd, gpu):
     L.smd_ps_get_positions(ps, p0.ctypes.data_as(vp), n)
     L.smd_ps_get_types(ps, t0.ctypes.data_as(vp), n)
     assert (t0 == 1).sum() == 3 * (n // 4)
-    assert np.all(p0[:, 3] == 1.0) and p0[:, :3].min() > -1.0 and p0[:, :3].max() < 1.0
+    # initGrid (particleSystem.cpp:1088-1117): s = ceil(n^(1/3)) points per
+    # axis at spacing 2 / (s - 2) from -1 + r, jittered by at most r / 100
+    # (the lattice runs past 1; update() clamps it)
+    r = np.float32(1 / 64.0)
+    s = int(np.ceil(np.float32(n) ** np.float32(1.0 / 3.0)))
+    i = np.arange(n)
+    lat = np.stack([i % s, (i // s) % s, i // (s * s)], axis=1).astype(np.float32)
+    want = lat * np.float32(2.0 / (s - 2)) + r - np.float32(1.0)
+    assert np.all(p0[:, 3] == 1.0)
+    assert np.abs(p0[:, :3] - want).max() <= r * np.float32(0.01) * 1.001
     spec = configs.make_spec(D=[0.0, 0.3, 0.1], dt=0.01, reactions=[], box_lo=[-1.0] * 3, box_hi=[1.0] * 3,
                              faces=[BC_CLAMP] * 6, grid_bits=(6, 6, 6), cell_size=[1 / 32.0] * 3,
                              origin=[-1.0] * 3, num_streams=32768, particle_radius=1 / 64.0, volume=90.0)
