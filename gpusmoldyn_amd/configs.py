@@ -8,6 +8,7 @@ each spec's `seed`.
   c2         1,048,576 particles, A + B <-> C and C -> 0, periodic 100 um cube
   c3         4,194,304 particles in 64 Gaussian blobs, A + B <-> C, reflective
              sphere r = 60 um, absorbing plane z = 0, 200 um cube
+  (c2 and c3 enter in hash-bucket order, as a lattice initialisation would)
   c4         512 replicas of c1's network, n log-uniform in [2048, 65536],
              kf log-uniform in [1e-4, 1e-2]
 """
@@ -63,6 +64,21 @@ def make_spec(D, dt, reactions, box_lo, box_hi, faces, grid_bits, cell_size=None
     return s
 
 
+def cell_order(spec, pos, types):
+    """The particles reordered by hash-grid bucket (stable), the order a
+    lattice initialisation (ParticleSystem::initGrid) produces: the step's
+    gathers into cell order then read nearby memory."""
+    from . import spec_cells
+
+    cell, origin = spec_cells(spec)
+    g = np.floor((pos[:, :3] - np.asarray(origin, np.float32)) / np.asarray(cell, np.float32)).astype(np.int64)
+    bits = [int(spec.grid_bits[d]) for d in range(3)]
+    key = ((g[:, 2] & ((1 << bits[2]) - 1)) << (bits[0] + bits[1])) | ((g[:, 1] & ((1 << bits[1]) - 1)) << bits[0]) \
+        | (g[:, 0] & ((1 << bits[0]) - 1))
+    order = np.argsort(key, kind="stable")
+    return np.ascontiguousarray(pos[order]), np.ascontiguousarray(types[order])
+
+
 def uniform_box(rng, n, lo, hi):
     p = np.empty((n, 4), np.float32)
     p[:, :3] = rng.uniform(lo, hi, size=(n, 3)).astype(np.float32)
@@ -100,13 +116,16 @@ def c2_spec(n_streams=131072, grid_bits=(7, 7, 7), L=100.0, dt=1e-3, seed=SEED):
         num_streams=n_streams, seed=seed, unbind_offsets=1)
 
 
-def c2(n=1 << 20, n_streams=131072, grid_bits=(7, 7, 7), L=100.0, dt=1e-3, seed=SEED):
+def c2(n=1 << 20, n_streams=131072, grid_bits=(7, 7, 7), L=100.0, dt=1e-3, seed=SEED, ordered=True):
     rng = np.random.default_rng(SEED)
     pos = uniform_box(rng, n, 0.0, L)
     types = np.empty(n, np.int32)
     types[: n // 2] = A
     types[n // 2:] = B
-    return c2_spec(n_streams, grid_bits, L, dt, seed), pos, types
+    spec = c2_spec(n_streams, grid_bits, L, dt, seed)
+    if ordered:
+        pos, types = cell_order(spec, pos, types)
+    return spec, pos, types
 
 
 C2_STEPS = 500
@@ -125,7 +144,8 @@ def c3_spec(n_streams=262144, grid_bits=(8, 8, 7), cell=0.1, L=200.0, seed=SEED)
         spheres=[(L / 2, L / 2, L / 2, 60.0)], unbind_offsets=1)
 
 
-def c3(n=1 << 22, n_blobs=64, sigma=2.0, n_streams=262144, grid_bits=(8, 8, 7), cell=0.1, L=200.0, seed=SEED):
+def c3(n=1 << 22, n_blobs=64, sigma=2.0, n_streams=262144, grid_bits=(8, 8, 7), cell=0.1, L=200.0, seed=SEED,
+       ordered=True):
     rng = np.random.default_rng(SEED)
     centres = rng.uniform(0.0, L, size=(n_blobs, 3))
     which = rng.integers(0, n_blobs, size=n)
@@ -138,7 +158,10 @@ def c3(n=1 << 22, n_blobs=64, sigma=2.0, n_streams=262144, grid_bits=(8, 8, 7), 
     pos[:, :3] = p.astype(np.float32)
     pos[:, 3] = 1.0
     types = (np.arange(n) % 3 + 1).astype(np.int32)
-    return c3_spec(n_streams, grid_bits, cell, L, seed), pos, types
+    spec = c3_spec(n_streams, grid_bits, cell, L, seed)
+    if ordered:
+        pos, types = cell_order(spec, pos, types)
+    return spec, pos, types
 
 
 C3_STEPS = 200

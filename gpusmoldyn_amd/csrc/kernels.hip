@@ -694,152 +694,66 @@ __device__ __forceinline__ void scan_range(const ReplicaTables& T, const float4*
   if (k < b) consider(T, spt[k], k, j, px, py, pz, tj, mask, maxB2, best, bestD2);
 }
 
-// One run [a, b) of sorted indices, read from the LDS chunk [cs, ce) when it
-// is staged there, else from global memory.
-__device__ __forceinline__ void scan_run(const ReplicaTables& T, const float4* __restrict__ spt,
-                                         const float4* chunk, uint32_t cs, bool staged, uint32_t a, uint32_t b,
-                                         uint32_t j, float px, float py, float pz, int tj, uint32_t mask,
-                                         float maxB2, int& best, float& bestD2) {
-  if (staged) {
-    for (uint32_t k = a; k < b; ++k)
-      consider(T, chunk[k - cs], k, j, px, py, pz, tj, mask, maxB2, best, bestD2);
-  } else {
-    scan_range(T, spt, a, b, j, px, py, pz, tj, mask, maxB2, best, bestD2);
-  }
-}
-
-// LDS-tiled 27-neighbour search.  A workgroup takes 256 consecutive sorted
-// particles of one replica (keys are non-decreasing in j).  For a (dz, dy)
-// neighbour-row offset that wraps for none of them, every particle's row is
-// its own row shifted by a constant, so the buckets needed by the whole
-// workgroup lie in the whole rows from the first particle's shifted row to the
-// last one's: one contiguous range of sorted particles.  Those ranges are
-// staged in LDS three at a time and each particle scans its bucket runs from
-// LDS.  Offsets that wrap at a grid edge, ranges longer than a chunk, and
-// particles of a second replica in the workgroup read global memory instead.
-constexpr uint32_t kCandChunk = 768;
-
-__device__ __forceinline__ uint32_t row_of(uint32_t key, const ReplicaTables& T, int q, bool& wraps) {
-  const uint32_t bx = T.gbits[0], by = T.gbits[1];
-  const uint32_t my = (1u << by) - 1u, mz = (1u << T.gbits[2]) - 1u;
-  const uint32_t local = key - T.bucketOff;
-  const int gy = (int)((local >> bx) & my) + q % 3 - 1;
-  const int gz = (int)(local >> (bx + by)) + q / 3 - 1;
-  wraps = gy < 0 || gy > (int)my || gz < 0 || gz > (int)mz;
-  return T.bucketOff + (((uint32_t)gz & mz) << (bx + by)) + (((uint32_t)gy & my) << bx);
-}
+// Face-aware neighbour search.  Every binding radius is at most the cell
+// size, so a partner k of j within the binding radius lies in a neighbouring
+// cell only across a face that j is within that radius of: along each axis
+// the search takes the cell below only when j is near the lower face, the
+// cell above only when it is near the upper face.  This visits a subset of
+// the 27 cells that contains every pair within the radius, so the nearest
+// partner (and every tie) is the one the full 27-cell scan finds.  The
+// near-face margin is widened by 1/64 of a cell, far above the rounding of
+// the cell coordinates, so no pair is ever lost to rounding.
+constexpr float kFaceMargin = 1.0f / 64.0f;
 
 __global__ __launch_bounds__(256) void k_candidates(CandArgs a) {
-  __shared__ float4 s_pt[3][kCandChunk];
-  __shared__ uint32_t s_cs[9];
-  __shared__ int s_len[9];  // staged length + 1, 0 = not staged
-  __shared__ uint32_t s_cnt[4], s_wrap[4];
+  const uint32_t j = blockIdx.x * 256 + threadIdx.x;
   const uint32_t nLive = *a.dLive;
-  const uint32_t j0 = blockIdx.x * 256;
-  if (j0 >= nLive) return;
-  const uint32_t j = j0 + threadIdx.x;
-  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
-  const bool active = j < nLive;
-  const uint32_t key0 = a.keys[j0];
-  const int r0 = replica_of_bucket(a.bucketStart, a.R, key0);
-
-  uint32_t key = key0;
-  int r = r0;
-  float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (active) {
-    key = a.keys[j];
-    r = replica_of_bucket(a.bucketStart, a.R, key);
-    p = a.sortedPT[j];
-  }
+  if (j >= nLive) return;
+  const uint32_t key = a.keys[j];
+  const int r = replica_of_bucket(a.bucketStart, a.R, key);
   const ReplicaTables& T = a.tables[r];
+  const float4 p = a.sortedPT[j];
   const int tj = __float_as_int(p.w);
-  const uint32_t mask = (active && T.hasBim && tj > 0 && tj < kMaxTypes) ? T.partnerMask[tj] : 0u;
-  const float maxB2 = mask ? T.maxBindR2[tj] : 0.0f;
-  const uint32_t bx = T.gbits[0];
-  const uint32_t mx = (1u << bx) - 1u;
-  const uint32_t gx = (key - T.bucketOff) & mx;
-  const bool inner = gx >= 1 && gx < mx;
-  uint32_t rowBase[9];
-  uint32_t wrapBits = 0;
-#pragma unroll
-  for (int q = 0; q < 9; ++q) {
-    bool wr;
-    rowBase[q] = row_of(key, T, q, wr);
-    wrapBits |= (wr ? 1u : 0u) << q;
-  }
-  // Run bounds of the inner case, all requested before any scanning.
-  const uint32_t* __restrict__ first = a.first;
-  uint32_t lo[9], hi[9];
-  if (mask && inner) {
-#pragma unroll
-    for (int q = 0; q < 9; ++q) {
-      lo[q] = first[rowBase[q] + gx - 1u];
-      hi[q] = first[rowBase[q] + gx + 2u];
-    }
-  }
-  // Workgroup extent in replica r0 (a prefix of the workgroup) and the union
-  // of wrapping offsets.
-  const bool isR0 = active && r == r0;
-  const uint64_t bal = __ballot(isR0);
-  uint32_t wb = isR0 ? wrapBits : 0u;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) wb |= __shfl_xor(wb, o, 64);
-  if (lane == 0) {
-    s_cnt[w] = (uint32_t)__popcll(bal);
-    s_wrap[w] = wb;
-  }
-  __syncthreads();
-  if (threadIdx.x < 9) {
-    const int q = (int)threadIdx.x;
-    const uint32_t nR0 = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
-    const uint32_t wrapAll = s_wrap[0] | s_wrap[1] | s_wrap[2] | s_wrap[3];
-    int len = 0;
-    uint32_t cs = 0;
-    if (nR0 > 0 && !((wrapAll >> q) & 1u)) {
-      const ReplicaTables& T0 = a.tables[r0];
-      bool wr;
-      const uint32_t b0 = row_of(key0, T0, q, wr);
-      const uint32_t b1 = row_of(a.keys[j0 + nR0 - 1u], T0, q, wr) + ((1u << T0.gbits[0]) - 1u);
-      cs = first[b0];
-      const uint32_t ce = first[b1 + 1u];
-      len = (ce - cs) <= kCandChunk ? (int)(ce - cs) + 1 : 0;
-    }
-    s_cs[q] = cs;
-    s_len[q] = len;
-  }
-  __syncthreads();
-
+  const uint32_t mask = (T.hasBim && tj > 0 && tj < kMaxTypes) ? T.partnerMask[tj] : 0u;
   int best = -1;
-  float bestD2 = 3.4e38f;
-  for (int ph = 0; ph < 3; ++ph) {
+  if (mask) {
+    const float maxB2 = T.maxBindR2[tj];
+    const float pc[3] = {p.x, p.y, p.z};
+    int lo[3], hi[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const int len = s_len[3 * ph + c] - 1;
-      const uint32_t cs = s_cs[3 * ph + c];
-      for (int k = (int)threadIdx.x; k < len; k += 256) s_pt[c][k] = a.sortedPT[cs + (uint32_t)k];
+    for (int d = 0; d < 3; ++d) {
+      const float q = (pc[d] - T.origin[d]) / T.cellSize[d];
+      const float f = q - floorf(q);
+      const float rb = sqrtf(maxB2) / T.cellSize[d] + kFaceMargin;
+      lo[d] = f < rb ? 1 : 0;
+      hi[d] = f > 1.0f - rb ? 1 : 0;
     }
-    __syncthreads();
-    if (mask) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const int q = 3 * ph + c;
-        const bool staged = isR0 && s_len[q] > 0;
-        const uint32_t cs = s_cs[q];
-        if (inner) {
-          scan_run(T, a.sortedPT, s_pt[c], cs, staged, lo[q], hi[q], j, p.x, p.y, p.z, tj, mask, maxB2, best,
-                   bestD2);
-        } else {
-          for (int dx = -1; dx <= 1; ++dx) {
-            const uint32_t xx = (gx + (uint32_t)dx) & mx;
-            scan_run(T, a.sortedPT, s_pt[c], cs, staged, first[rowBase[q] + xx], first[rowBase[q] + xx + 1u], j,
+    float bestD2 = 3.4e38f;
+    const uint32_t local = key - T.bucketOff;
+    const uint32_t bx = T.gbits[0], by = T.gbits[1];
+    const uint32_t mx = (1u << bx) - 1u, my = (1u << by) - 1u, mz = (1u << T.gbits[2]) - 1u;
+    const uint32_t gx = local & mx, gy = (local >> bx) & my, gz = local >> (bx + by);
+    const uint32_t* __restrict__ first = a.first;
+    const bool xRun = gx >= (uint32_t)lo[0] && gx + (uint32_t)hi[0] <= mx;  // x range without wrap
+    for (int dz = -lo[2]; dz <= hi[2]; ++dz) {
+      const uint32_t zz = (gz + (uint32_t)dz) & mz;
+      for (int dy = -lo[1]; dy <= hi[1]; ++dy) {
+        const uint32_t yy = (gy + (uint32_t)dy) & my;
+        const uint32_t row = T.bucketOff + (zz << (bx + by)) + (yy << bx);
+        if (xRun) {
+          scan_range(T, a.sortedPT, first[row + gx - (uint32_t)lo[0]], first[row + gx + (uint32_t)hi[0] + 1u], j,
                      p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
+        } else {
+          for (int dx = -lo[0]; dx <= hi[0]; ++dx) {
+            const uint32_t xx = (gx + (uint32_t)dx) & mx;
+            scan_range(T, a.sortedPT, first[row + xx], first[row + xx + 1u], j, p.x, p.y, p.z, tj, mask, maxB2,
+                       best, bestD2);
           }
         }
       }
     }
-    __syncthreads();
   }
-  if (active) a.cand[j] = best;
+  a.cand[j] = best;
 }
 
 // ---------------------------------------------------------------------------
