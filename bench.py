@@ -179,6 +179,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="c4: skip the one-replica-per-launch comparison")
     ap.add_argument("--no-graphs", action="store_true", help="one launch sequence per step instead of graphs")
+    ap.add_argument("--sort", type=int, default=0, help="0: single-sweep radix sort, 1: count/scan/scatter")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--jobs", type=int, default=200, help="c5: number of jobs")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -200,6 +201,8 @@ def main():
     e = make_engine(reps, args.radix_bits)
     if args.no_graphs:
         e.set_graphs(False)
+    if args.sort:
+        e.set_sort(args.sort)
     e.step(args.warmup)
     ps, secs = timed(e, steps)
     rl, step = roofline(e, reps)

@@ -97,6 +97,7 @@ def lib():
         L.smd_engine_counts.argtypes = [vp, u32p]
         L.smd_engine_particle_steps.argtypes = [vp, ctypes.POINTER(ctypes.c_ulonglong)]
         L.smd_engine_set_graphs.argtypes = [vp, ctypes.c_int]
+        L.smd_engine_set_sort.argtypes = [vp, ctypes.c_int]
         L.smd_engine_num_replicas.argtypes = [vp]
         L.smd_engine_num_replicas.restype = ctypes.c_uint
         L.smd_engine_get.argtypes = [vp, ctypes.c_int, f32p, i32p, ctypes.c_uint, ctypes.POINTER(ctypes.c_uint)]
@@ -287,6 +288,10 @@ class Engine:
         out = np.zeros(R, np.uint32)
         _check(lib().smd_engine_counts(self._h, _ptr(out)), "smd_engine_counts")
         return out
+
+    def set_sort(self, kind):
+        """0: single-sweep radix sort (default), 1: count / scan / scatter per digit."""
+        _check(lib().smd_engine_set_sort(self._h, int(kind)), "smd_engine_set_sort")
 
     def set_graphs(self, on):
         _check(lib().smd_engine_set_graphs(self._h, int(bool(on))), "smd_engine_set_graphs")

@@ -236,6 +236,13 @@ smd_status smd_engine_set_graphs(smd_engine* e, int on) {
   return SMD_OK;
 }
 
+smd_status smd_engine_set_sort(smd_engine* e, int kind) {
+  if (!e || (kind != 0 && kind != 1)) return SMD_ERR_INVALID;
+  e->impl->drop_graph();
+  e->impl->sb.algo = kind;
+  return SMD_OK;
+}
+
 smd_status smd_engine_particle_steps(smd_engine* e, unsigned long long* out) {
   if (!e || !out) return SMD_ERR_INVALID;
   return e->impl->particle_steps(out);
@@ -348,6 +355,7 @@ smd_status smd_sort_pairs(int device, uint32_t* keys, uint32_t* values, unsigned
   CAPI_HIPCHK(hipSetDevice(device));
   smd::SortBuffers b;
   b.status = nullptr;
+  b.algo = 0;
   uint32_t* dN = nullptr;
   CAPI_HIPCHK(hipMalloc((void**)&b.keys, (size_t)n * 4));
   CAPI_HIPCHK(hipMalloc((void**)&b.vals, (size_t)n * 4));

@@ -1134,6 +1134,9 @@ size_t sort_counts_words(uint32_t maxN, int radixBits) {
   return a > b ? a : b;
 }
 
+hipError_t launch_sort_3k(SortBuffers& b, const uint32_t* dN, uint32_t maxN, uint32_t keyBits, int radixBits,
+                          bool identityVals, hipStream_t stream);
+
 static hipError_t launch_sort_onesweep(SortBuffers& b, const uint32_t* dN, uint32_t maxN, uint32_t keyBits,
                                        int radixBits, bool identityVals, hipStream_t stream) {
   const uint32_t maxTiles = cdiv(maxN, kSortTile);
@@ -1175,6 +1178,7 @@ static hipError_t launch_sort_onesweep(SortBuffers& b, const uint32_t* dN, uint3
 
 hipError_t launch_sort(SortBuffers& b, const uint32_t* dN, uint32_t maxN, uint32_t keyBits, int radixBits,
                        bool identityVals, hipStream_t stream) {
+  if (b.algo == 1) return launch_sort_3k(b, dN, maxN, keyBits, radixBits, identityVals, stream);
   return launch_sort_onesweep(b, dN, maxN, keyBits, radixBits, identityVals, stream);
 }
 

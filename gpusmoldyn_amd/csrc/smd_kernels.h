@@ -36,6 +36,7 @@ struct SortBuffers {
   uint32_t* valsAlt;
   uint32_t* counts;  // sort_counts_words(maxN, radixBits)
   uint32_t* status;  // sticky status word (kStatusSortSpin), or nullptr
+  int algo;          // 0: single sweep with look-back, 1: count / scan / scatter per digit
 };
 
 struct CandArgs {

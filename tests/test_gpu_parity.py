@@ -330,3 +330,18 @@ def test_autotune_picks_a_valid_fastest_config(smd, gpu):
     e.step(20)
     c = e.species_counts(0)
     assert c[1] + c[3] == (types == 1).sum() and c[2] + c[3] == (types == 2).sum()
+
+
+def test_sort_kinds_agree(smd, gpu):
+    """The count/scan/scatter sort and the single-sweep sort give the same
+    stable order, hence the same particles step after step."""
+    spec, pos, types = w_periodic_reversible()
+    banner("sort kinds")
+    a = _engine(smd, spec, pos, types)
+    b = _engine(smd, spec, pos, types)
+    b.set_sort(1)
+    a.step(15)
+    b.step(15)
+    ap, at = a.get(0)
+    bp, bt = b.get(0)
+    assert np.array_equal(at, bt) and np.array_equal(ap.view(np.uint32), bp.view(np.uint32))
