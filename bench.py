@@ -179,7 +179,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="c4: skip the one-replica-per-launch comparison")
     ap.add_argument("--no-graphs", action="store_true", help="one launch sequence per step instead of graphs")
-    ap.add_argument("--sort", type=int, default=0, help="0: single-sweep radix sort, 1: count/scan/scatter")
+    ap.add_argument("--sort", type=int, default=1, help="1: count/scan/scatter (default), 0: single-sweep radix sort")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--jobs", type=int, default=200, help="c5: number of jobs")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -201,7 +201,7 @@ def main():
     e = make_engine(reps, args.radix_bits)
     if args.no_graphs:
         e.set_graphs(False)
-    if args.sort:
+    if args.sort != 1:
         e.set_sort(args.sort)
     e.step(args.warmup)
     ps, secs = timed(e, steps)

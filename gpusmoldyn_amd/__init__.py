@@ -290,7 +290,7 @@ class Engine:
         return out
 
     def set_sort(self, kind):
-        """0: single-sweep radix sort (default), 1: count / scan / scatter per digit."""
+        """1: count / scan / scatter per digit (default), 0: single-sweep radix sort."""
         _check(lib().smd_engine_set_sort(self._h, int(kind)), "smd_engine_set_sort")
 
     def set_graphs(self, on):

@@ -355,7 +355,7 @@ smd_status smd_sort_pairs(int device, uint32_t* keys, uint32_t* values, unsigned
   CAPI_HIPCHK(hipSetDevice(device));
   smd::SortBuffers b;
   b.status = nullptr;
-  b.algo = 0;
+  b.algo = 1;
   uint32_t* dN = nullptr;
   CAPI_HIPCHK(hipMalloc((void**)&b.keys, (size_t)n * 4));
   CAPI_HIPCHK(hipMalloc((void**)&b.vals, (size_t)n * 4));

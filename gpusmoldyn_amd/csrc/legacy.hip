@@ -751,7 +751,7 @@ int radixSort(unsigned int* keys, unsigned int* values, unsigned int* tempKeys, 
   b.valsAlt = tempValues;
   b.counts = dN + 1;
   b.status = nullptr;
-  b.algo = 0;
+  b.algo = 1;
   LCHK(hipMemcpy(dN, &numElements, sizeof(uint32_t), hipMemcpyHostToDevice));
   LCHK(launch_sort(b, dN, numElements, keyBits, rbits, false, nullptr));
   if (b.keys != keys) {

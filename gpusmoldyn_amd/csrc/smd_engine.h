@@ -69,7 +69,7 @@ class Engine {
   int* dBirthType = nullptr;
   float4* dSortedPT = nullptr;
   int* dCand = nullptr;
-  SortBuffers sb = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0};
+  SortBuffers sb = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 1};
   uint32_t *dTileAlive = nullptr, *dTileBirth = nullptr, *dTileAlivePre = nullptr, *dTileBirthPre = nullptr;
   uint32_t* dStatus = nullptr;  // sticky kernel error bits
   // order-0 creation: one stream per replica, a per-step creation buffer

@@ -173,9 +173,9 @@ smd_status smd_engine_step(smd_engine* e, int nsteps);
 /* Replay pairs of steps as one captured hipGraph (default on); off = one
  * launch sequence per step. */
 smd_status smd_engine_set_graphs(smd_engine* e, int on);
-/* Key sort of the step: 0 = single sweep (one histogram kernel, one binning
- * kernel per digit with decoupled look-back; default), 1 = count, scan and
- * scatter kernels per digit.  Both give the same stable order. */
+/* Key sort of the step: 1 = count, scan and scatter kernels per digit
+ * (default), 0 = single sweep (one histogram kernel, one binning kernel per
+ * digit with decoupled look-back).  Both give the same stable order. */
 smd_status smd_engine_set_sort(smd_engine* e, int kind);
 smd_status smd_engine_sync(smd_engine* e);
 /* Current particle count of each replica (synchronises). */

@@ -339,7 +339,7 @@ def test_sort_kinds_agree(smd, gpu):
     banner("sort kinds")
     a = _engine(smd, spec, pos, types)
     b = _engine(smd, spec, pos, types)
-    b.set_sort(1)
+    b.set_sort(0)
     a.step(15)
     b.step(15)
     ap, at = a.get(0)
