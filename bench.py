@@ -34,13 +34,14 @@ def stage_bytes(name, n, S, nb, passes):
     """Algorithmic bytes of one launch of each stage for n particles, S RNG
     streams, nb buckets (compulsory reads + writes, each datum once)."""
     if name == "producer":
-        return 48 * n + 172 * S
+        # pos 16 + type 4 in; pos 16, type 4, key 4, birth type 4, packed record 16 out; stream state
+        return 64 * n + 172 * S
     if name == "sort":
-        return 16 * n + 20 * n * (passes - 1)
+        # per digit pass: count (key 4) + scatter (key, value 8 in, 8 out); first pass values implicit;
+        # the last pass also copies the packed record of each particle into sorted order (16 + 16)
+        return 20 * n * passes - 4 * n + 32 * n
     if name == "first":
         return 4 * (nb + 1) + 4 * n
-    if name == "gather":
-        return 44 * n
     if name == "candidates":
         return 24 * n
     if name == "resolve":

@@ -356,6 +356,9 @@ smd_status smd_sort_pairs(int device, uint32_t* keys, uint32_t* values, unsigned
   smd::SortBuffers b;
   b.status = nullptr;
   b.algo = 1;
+  b.gatherSrc = nullptr;
+  b.gatherDst = nullptr;
+  b.gatherDeadKey = 0;
   uint32_t* dN = nullptr;
   CAPI_HIPCHK(hipMalloc((void**)&b.keys, (size_t)n * 4));
   CAPI_HIPCHK(hipMalloc((void**)&b.vals, (size_t)n * 4));

@@ -93,6 +93,7 @@ void Engine::free_particle_buffers() {
   dfree(dAddedPos);
   dfree(dBirthType);
   dfree(dSortedPT);
+  dfree(dPType);
   dfree(dCand);
   dfree(sb.keys);
   dfree(sb.vals);
@@ -139,6 +140,7 @@ smd_status Engine::alloc_particle_buffers(uint32_t c) {
   HIPCHK(dalloc(&dAddedPos, cap));
   HIPCHK(dalloc(&dBirthType, cap));
   HIPCHK(dalloc(&dSortedPT, cap));
+  HIPCHK(dalloc(&dPType, cap));
   HIPCHK(dalloc(&dCand, cap));
   HIPCHK(dalloc(&sb.keys, cap));
   HIPCHK(dalloc(&sb.vals, cap));
@@ -481,6 +483,7 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
   pa.tileAlive = dTileAlive;
   pa.tileBirth = dTileBirth;
   pa.retire = dRetire;
+  pa.ptype = dPType;
   HIPCHK(launch_producer(pa, stream));
   if (anyZero) {
     ZeroArgs za;
@@ -496,12 +499,13 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
   }
   mark(1);
   if (anyBim) {
+    sb.gatherSrc = dPType;
+    sb.gatherDst = dSortedPT;
+    sb.gatherDeadKey = deadKey;
     HIPCHK(launch_sort(sb, dN, maxN, keyBits, radixBits, true, stream));
     mark(2);
     HIPCHK(launch_first(sb.keys, dN, nb, dFirst, dLive, stream));
     mark(3);
-    HIPCHK(launch_gather(sb.keys, sb.vals, dPos[cur], dType[cur], dN, maxN, deadKey, dSortedPT, stream));
-    mark(4);
     CandArgs ca;
     ca.tables = dTables;
     ca.bucketStart = dBucketStart;
@@ -512,7 +516,7 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
     ca.dLive = dLive;
     ca.cand = dCand;
     HIPCHK(launch_candidates(ca, maxN, stream));
-    mark(5);
+    mark(4);
     ResolveArgs ra;
     ra.tables = dTables;
     ra.streamReplica = dStreamReplica;
@@ -528,9 +532,9 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
     ra.repDeaths = dRepDeaths;
     ra.tileAlive = dTileAlive;
     HIPCHK(launch_resolve(ra, stream));
-    mark(6);
+    mark(5);
   } else {
-    for (int s = 2; s <= 6; ++s) mark(s);
+    for (int s = 2; s <= 5; ++s) mark(s);
   }
   FinalizeArgs fa;
   fa.R = (int)R;
@@ -555,7 +559,7 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
   fa.injBase = dInjBase;
   fa.retire = dRetire;
   HIPCHK(launch_finalize(fa, stream));
-  mark(7);
+  mark(6);
   ScatterArgs sa;
   sa.R = (int)R;
   sa.dN = dN;
@@ -592,7 +596,7 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
   }
   // every admission (empty ones too) has entered with this step
   std::fill(slotPending.begin(), slotPending.end(), (uint8_t)0);
-  mark(8);
+  mark(7);
   cur ^= 1;
   std::swap(dRepOff, dNewOff);
   std::swap(dRepCount, dNewCount);
@@ -926,12 +930,15 @@ smd_status Engine::debug_stage(int stage, float* pos4, int* types, uint32_t* key
   pa.tileAlive = dTileAlive;
   pa.tileBirth = dTileBirth;
   pa.retire = nullptr;
+  pa.ptype = dPType;
   HIPCHK(launch_producer(pa, stream));
   uint32_t live = 0;
   if (stage >= 2) {
+    sb.gatherSrc = dPType;
+    sb.gatherDst = dSortedPT;
+    sb.gatherDeadKey = deadKey;
     HIPCHK(launch_sort(sb, dN, maxN, keyBits, radixBits, true, stream));
     HIPCHK(launch_first(sb.keys, dN, nb, dFirst, dLive, stream));
-    HIPCHK(launch_gather(sb.keys, sb.vals, dPos[cur], dType[cur], dN, maxN, deadKey, dSortedPT, stream));
   }
   if (stage >= 3) {
     CandArgs ca;
@@ -980,30 +987,29 @@ smd_status Engine::debug_stage(int stage, float* pos4, int* types, uint32_t* key
 }
 
 smd_status Engine::profile(int nsteps, double* us, int maxStages, char* names, int namesCap, int* nStages) {
-  static const char* kNames[8] = {"producer", "sort", "first", "gather", "candidates", "resolve", "finalize",
-                                  "scatter"};
+  static const char* kNames[7] = {"producer", "sort", "first", "candidates", "resolve", "finalize", "scatter"};
   if (!finalized || broken) {
     set_error("engine not usable");
     return SMD_ERR_INVALID;
   }
   HIPCHK(hipSetDevice(device));
   StageTimer tm;
-  for (int k = 0; k < 9; ++k) HIPCHK(hipEventCreate(&tm.ev[k]));
-  std::vector<double> acc(8, 0.0);
+  for (int k = 0; k < 8; ++k) HIPCHK(hipEventCreate(&tm.ev[k]));
+  std::vector<double> acc(7, 0.0);
   for (int s = 0; s < nsteps; ++s) {
     smd_status st = ensure_capacity();
     if (st != SMD_OK) return st;
     st = enqueue_step(&tm);
     if (st != SMD_OK) return st;
-    HIPCHK(hipEventSynchronize(tm.ev[8]));
-    for (int k = 0; k < 8; ++k) {
+    HIPCHK(hipEventSynchronize(tm.ev[7]));
+    for (int k = 0; k < 7; ++k) {
       float ms = 0;
       HIPCHK(hipEventElapsedTime(&ms, tm.ev[k], tm.ev[k + 1]));
       acc[k] += ms * 1000.0;
     }
   }
-  for (int k = 0; k < 9; ++k) (void)hipEventDestroy(tm.ev[k]);
-  const int m = std::min(8, maxStages);
+  for (int k = 0; k < 8; ++k) (void)hipEventDestroy(tm.ev[k]);
+  const int m = std::min(7, maxStages);
   for (int k = 0; k < m; ++k) us[k] = nsteps > 0 ? acc[k] / nsteps : 0.0;
   if (nStages) *nStages = m;
   if (names && namesCap > 0) {

@@ -119,6 +119,7 @@ __global__ __launch_bounds__(kBS) void k_producer(ProducerArgs a) {
         a.type[i] = 0;
         a.key[i] = a.deadKey;
         a.birthType[i] = 0;
+        a.ptype[i] = make_float4(pCur.x, pCur.y, pCur.z, __int_as_float(0));
         ++nDeaths;
       } else if (act) {
         float4 p = pCur;
@@ -182,6 +183,7 @@ __global__ __launch_bounds__(kBS) void k_producer(ProducerArgs a) {
         }
         a.pos[i] = make_float4(np[0], np[1], np[2], w);
         a.type[i] = t;
+        a.ptype[i] = make_float4(np[0], np[1], np[2], __int_as_float(t));
         a.key[i] = key;
         a.birthType[i] = bt;
         if (bt) a.addedPos[i] = make_float4(bp[0], bp[1], bp[2], bw);
@@ -287,14 +289,15 @@ __global__ __launch_bounds__(256) void k_sort_rowscan(uint32_t* __restrict__ cou
   if (threadIdx.x == 0) totals[d] = carry;
 }
 
-template <int BITS>
+template <int BITS, bool GATHER>
 __global__ __launch_bounds__(256) void k_sort_down(const uint32_t* __restrict__ keysIn,
                                                    const uint32_t* __restrict__ valsIn,
                                                    uint32_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut,
                                                    const uint32_t* __restrict__ counts,
                                                    const uint32_t* __restrict__ totals,
                                                    const uint32_t* __restrict__ dN, uint32_t stride, int shift,
-                                                   int identityVals) {
+                                                   int identityVals, const float4* __restrict__ gatherSrc,
+                                                   float4* __restrict__ gatherDst, uint32_t gatherDeadKey) {
   constexpr int RADIX = 1 << BITS;
   constexpr int DPT = (RADIX + 255) / 256;  // digits per thread in the per-digit loops
   __shared__ uint32_t s_base[RADIX];
@@ -397,8 +400,10 @@ __global__ __launch_bounds__(256) void k_sort_down(const uint32_t* __restrict__ 
     const uint32_t k = s_key[idx];
     const uint32_t d = (k >> shift) & (RADIX - 1);
     const uint32_t dst = s_base[d] + (idx - s_tstart[d]);
+    const uint32_t v = s_val[idx];
     keysOut[dst] = k;
-    valsOut[dst] = s_val[idx];
+    valsOut[dst] = v;
+    if (GATHER && k < gatherDeadKey) gatherDst[dst] = gatherSrc[v];
   }
 }
 
@@ -633,17 +638,14 @@ __global__ __launch_bounds__(256) void k_first(const uint32_t* __restrict__ keys
   if (h1 == nb && threadIdx.x == 0) first[nb] = n;
 }
 
-// sortedPT[j] = (pos[val[j]].xyz, type[val[j]] as float bits) for live j.
+// sortedPT[j] = src[val[j]] (x, y, z, type bits) for live j.
 __global__ __launch_bounds__(256) void k_gather(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                                                const float4* __restrict__ pos, const int* __restrict__ type,
-                                                const uint32_t* __restrict__ dN, uint32_t deadKey,
-                                                float4* __restrict__ sortedPT) {
+                                                const float4* __restrict__ src, const uint32_t* __restrict__ dN,
+                                                uint32_t deadKey, float4* __restrict__ sortedPT) {
   const uint32_t j = blockIdx.x * 256 + threadIdx.x;
   if (j >= *dN) return;
   if (keys[j] >= deadKey) return;
-  const uint32_t i = vals[j];
-  const float4 p = pos[i];
-  sortedPT[j] = make_float4(p.x, p.y, p.z, __int_as_float(type[i]));
+  sortedPT[j] = src[vals[j]];
 }
 
 // ---------------------------------------------------------------------------
@@ -1173,6 +1175,9 @@ static hipError_t launch_sort_onesweep(SortBuffers& b, const uint32_t* dN, uint3
     std::swap(b.keys, b.keysAlt);
     std::swap(b.vals, b.valsAlt);
   }
+  if (b.gatherDst)
+    hipLaunchKernelGGL(k_gather, dim3(maxTiles * (kSortTile / 256)), dim3(256), 0, stream, b.keys, b.vals,
+                       b.gatherSrc, dN, b.gatherDeadKey, b.gatherDst);
   return hipGetLastError();
 }
 
@@ -1193,16 +1198,27 @@ hipError_t launch_sort_3k(SortBuffers& b, const uint32_t* dN, uint32_t maxN, uin
   for (int p = 0; p < passes; ++p) {
     const int shift = p * radixBits;
     const bool idv = identityVals && p == 0;
+    const bool gather = b.gatherDst != nullptr && p == passes - 1;
     if (radixBits == 8) {
       hipLaunchKernelGGL(k_sort_up<8>, dim3(maxTiles), dim3(256), 0, stream, b.keys, dN, b.counts, stride, shift);
       hipLaunchKernelGGL(k_sort_rowscan<8>, dim3(radix), dim3(256), 0, stream, b.counts, totals, dN, stride);
-      hipLaunchKernelGGL(k_sort_down<8>, dim3(maxTiles), dim3(256), 0, stream, b.keys, b.vals, b.keysAlt,
-                         b.valsAlt, b.counts, totals, dN, stride, shift, idv ? 1 : 0);
+      if (gather)
+        hipLaunchKernelGGL((k_sort_down<8, true>), dim3(maxTiles), dim3(256), 0, stream, b.keys, b.vals, b.keysAlt,
+                           b.valsAlt, b.counts, totals, dN, stride, shift, idv ? 1 : 0, b.gatherSrc, b.gatherDst,
+                           b.gatherDeadKey);
+      else
+        hipLaunchKernelGGL((k_sort_down<8, false>), dim3(maxTiles), dim3(256), 0, stream, b.keys, b.vals, b.keysAlt,
+                           b.valsAlt, b.counts, totals, dN, stride, shift, idv ? 1 : 0, nullptr, nullptr, 0u);
     } else if (radixBits == 4) {
       hipLaunchKernelGGL(k_sort_up<4>, dim3(maxTiles), dim3(256), 0, stream, b.keys, dN, b.counts, stride, shift);
       hipLaunchKernelGGL(k_sort_rowscan<4>, dim3(radix), dim3(256), 0, stream, b.counts, totals, dN, stride);
-      hipLaunchKernelGGL(k_sort_down<4>, dim3(maxTiles), dim3(256), 0, stream, b.keys, b.vals, b.keysAlt,
-                         b.valsAlt, b.counts, totals, dN, stride, shift, idv ? 1 : 0);
+      if (gather)
+        hipLaunchKernelGGL((k_sort_down<4, true>), dim3(maxTiles), dim3(256), 0, stream, b.keys, b.vals, b.keysAlt,
+                           b.valsAlt, b.counts, totals, dN, stride, shift, idv ? 1 : 0, b.gatherSrc, b.gatherDst,
+                           b.gatherDeadKey);
+      else
+        hipLaunchKernelGGL((k_sort_down<4, false>), dim3(maxTiles), dim3(256), 0, stream, b.keys, b.vals, b.keysAlt,
+                           b.valsAlt, b.counts, totals, dN, stride, shift, idv ? 1 : 0, nullptr, nullptr, 0u);
     } else {
       return hipErrorInvalidValue;
     }
@@ -1222,11 +1238,10 @@ hipError_t launch_first(const uint32_t* keys, const uint32_t* dN, uint32_t nb, u
   return hipGetLastError();
 }
 
-hipError_t launch_gather(const uint32_t* keys, const uint32_t* vals, const float4* pos, const int* type,
-                         const uint32_t* dN, uint32_t maxN, uint32_t deadKey, float4* sortedPT, hipStream_t stream) {
+hipError_t launch_gather(const uint32_t* keys, const uint32_t* vals, const float4* src, const uint32_t* dN,
+                         uint32_t maxN, uint32_t deadKey, float4* sortedPT, hipStream_t stream) {
   if (maxN == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_gather, dim3(cdiv(maxN, 256)), dim3(256), 0, stream, keys, vals, pos, type, dN, deadKey,
-                     sortedPT);
+  hipLaunchKernelGGL(k_gather, dim3(cdiv(maxN, 256)), dim3(256), 0, stream, keys, vals, src, dN, deadKey, sortedPT);
   return hipGetLastError();
 }
 

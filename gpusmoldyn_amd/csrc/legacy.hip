@@ -752,6 +752,9 @@ int radixSort(unsigned int* keys, unsigned int* values, unsigned int* tempKeys, 
   b.counts = dN + 1;
   b.status = nullptr;
   b.algo = 1;
+  b.gatherSrc = nullptr;
+  b.gatherDst = nullptr;
+  b.gatherDeadKey = 0;
   LCHK(hipMemcpy(dN, &numElements, sizeof(uint32_t), hipMemcpyHostToDevice));
   LCHK(launch_sort(b, dN, numElements, keyBits, rbits, false, nullptr));
   if (b.keys != keys) {

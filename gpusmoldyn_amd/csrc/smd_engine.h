@@ -11,7 +11,7 @@
 namespace smd {
 
 struct StageTimer {
-  hipEvent_t ev[9];
+  hipEvent_t ev[8];
 };
 
 class Engine {
@@ -68,8 +68,9 @@ class Engine {
   float4* dAddedPos = nullptr;
   int* dBirthType = nullptr;
   float4* dSortedPT = nullptr;
+  float4* dPType = nullptr;  // (x, y, z, type bits) after the producer, canonical order
   int* dCand = nullptr;
-  SortBuffers sb = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 1};
+  SortBuffers sb = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 1, nullptr, nullptr, 0};
   uint32_t *dTileAlive = nullptr, *dTileBirth = nullptr, *dTileAlivePre = nullptr, *dTileBirthPre = nullptr;
   uint32_t* dStatus = nullptr;  // sticky kernel error bits
   // order-0 creation: one stream per replica, a per-step creation buffer

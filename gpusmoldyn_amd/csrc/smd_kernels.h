@@ -27,6 +27,7 @@ struct ProducerArgs {
   uint32_t* tileAlive;
   uint32_t* tileBirth;
   const uint32_t* retire;     // per replica: remove every particle this step (nullptr: none)
+  float4* ptype;              // (x, y, z, type bits) after the move, canonical order (pair-search input)
 };
 
 struct SortBuffers {
@@ -37,6 +38,11 @@ struct SortBuffers {
   uint32_t* counts;  // sort_counts_words(maxN, radixBits)
   uint32_t* status;  // sticky status word (kStatusSortSpin), or nullptr
   int algo;          // 0: single sweep with look-back, 1: count / scan / scatter per digit
+  // optional gather into sorted order: gatherDst[j] = gatherSrc[vals[j]] for
+  // keys[j] < gatherDeadKey (fused into the last digit pass when possible)
+  const float4* gatherSrc;
+  float4* gatherDst;
+  uint32_t gatherDeadKey;
 };
 
 struct CandArgs {
@@ -141,8 +147,8 @@ hipError_t launch_sort(SortBuffers& b, const uint32_t* dN, uint32_t maxN, uint32
                        bool identityVals, hipStream_t stream);
 hipError_t launch_first(const uint32_t* keys, const uint32_t* dN, uint32_t nb, uint32_t* first, uint32_t* dLive,
                         hipStream_t stream);
-hipError_t launch_gather(const uint32_t* keys, const uint32_t* vals, const float4* pos, const int* type,
-                         const uint32_t* dN, uint32_t maxN, uint32_t deadKey, float4* sortedPT, hipStream_t stream);
+hipError_t launch_gather(const uint32_t* keys, const uint32_t* vals, const float4* src, const uint32_t* dN,
+                         uint32_t maxN, uint32_t deadKey, float4* sortedPT, hipStream_t stream);
 hipError_t launch_candidates(const CandArgs& a, uint32_t maxN, hipStream_t stream);
 hipError_t launch_resolve(const ResolveArgs& a, hipStream_t stream);
 hipError_t launch_finalize(const FinalizeArgs& a, hipStream_t stream);
