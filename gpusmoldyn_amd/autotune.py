@@ -33,11 +33,10 @@ def _candidates(spec, max_delta=2):
     return out
 
 
-def time_config(spec, pos, types, grid_bits, radix_bits, steps=20, warmup=3, device=0):
-    """Average wall time (us) of one step with the given grid and digits."""
-    s = copy.deepcopy(spec)
-    for d in range(3):
-        s.grid_bits[d] = grid_bits[d]
+def time_config(spec, pos, types, grid_bits, radix_bits, steps=20, warmup=3, device=0, cell=None):
+    """Average wall time (us) of one step with the given grid, cell size and
+    digits."""
+    s = apply(spec, grid_bits, cell)
     e = Engine(device, radix_bits)
     try:
         e.add_replica(s, pos, types)
@@ -52,24 +51,33 @@ def time_config(spec, pos, types, grid_bits, radix_bits, steps=20, warmup=3, dev
         e.close()
 
 
-def tune(spec, pos, types, steps=20, radix=(8, 4), max_delta=2, device=0):
-    """Returns ((grid_bits, radix_bits), table) with table[(grid_bits,
-    radix_bits)] = us per step; configurations the engine rejects are left out."""
+def tune(spec, pos, types, steps=20, radix=(8, 4), max_delta=2, device=0, cells=None, grids=None):
+    """Returns ((grid_bits, radix_bits, cell), table) with table[key] = us per
+    step; configurations the engine rejects are left out.  Without `cells`
+    the cell size follows the box (or stays the spec's explicit one) and the
+    grids are the spec's scaled by -max_delta..max_delta bits per axis;
+    `cells` (explicit cell sizes) and `grids` (grid_bits triples) give the
+    candidates directly."""
+    grid_list = [tuple(g) for g in grids] if grids else _candidates(spec, max_delta)
     table = {}
-    for bits in _candidates(spec, max_delta):
-        for rb in radix:
-            try:
-                table[(bits, rb)] = time_config(spec, pos, types, bits, rb, steps, device=device)
-            except SmdError:
-                continue
+    for cell in (cells or [None]):
+        for bits in grid_list:
+            for rb in radix:
+                try:
+                    table[(bits, rb, cell)] = time_config(spec, pos, types, bits, rb, steps, device=device, cell=cell)
+                except SmdError:
+                    continue
     if not table:
         raise SmdError("no valid grid candidate")
     best = min(table, key=table.get)
     return best, table
 
 
-def apply(spec, grid_bits):
+def apply(spec, grid_bits, cell=None):
+    """A copy of spec with the given grid (and explicit cell size)."""
     s = copy.deepcopy(spec)
     for d in range(3):
         s.grid_bits[d] = grid_bits[d]
+        if cell is not None:
+            s.cell_size[d] = cell
     return s

@@ -239,7 +239,7 @@ smd_status Engine::finalize(uint32_t capacity) {
   dLive = dCounters + 2;
   const uint32_t tot32 = (uint32_t)total;
   HIPCHK(hipMemcpy(dN, &tot32, sizeof(uint32_t), hipMemcpyHostToDevice));
-  HIPCHK(dalloc(&dFirst, (size_t)nb + 1));
+  HIPCHK(dalloc(&dFirst, first_words(nb)));
 
   // Gaussian table.
   std::vector<float> g(kGaussDim);
@@ -504,7 +504,7 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
     sb.gatherDeadKey = deadKey;
     HIPCHK(launch_sort(sb, dN, maxN, keyBits, radixBits, true, stream));
     mark(2);
-    HIPCHK(launch_first(sb.keys, dN, nb, dFirst, dLive, stream));
+    HIPCHK(launch_first(sb.keys, dN, maxN, nb, dFirst, dLive, stream));
     mark(3);
     CandArgs ca;
     ca.tables = dTables;
@@ -938,7 +938,7 @@ smd_status Engine::debug_stage(int stage, float* pos4, int* types, uint32_t* key
     sb.gatherDst = dSortedPT;
     sb.gatherDeadKey = deadKey;
     HIPCHK(launch_sort(sb, dN, maxN, keyBits, radixBits, true, stream));
-    HIPCHK(launch_first(sb.keys, dN, nb, dFirst, dLive, stream));
+    HIPCHK(launch_first(sb.keys, dN, maxN, nb, dFirst, dLive, stream));
   }
   if (stage >= 3) {
     CandArgs ca;

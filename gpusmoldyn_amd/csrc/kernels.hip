@@ -603,18 +603,32 @@ __device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* __restrict__
 constexpr uint32_t kFirstSpan = 4096;
 constexpr uint32_t kFirstStage = 8192;
 
+// blockStart[B] = lower bound of bucket B * kFirstSpan in the sorted keys,
+// for every B with B * kFirstSpan <= nb: element j writes the span starts that
+// lie in (keys[j - 1], keys[j]] (keys[-1] = -1, keys[n] = nb).
+__global__ __launch_bounds__(256) void k_first_blocks(const uint32_t* __restrict__ keys,
+                                                      const uint32_t* __restrict__ dN, uint32_t maxN, uint32_t nb,
+                                                      uint32_t* __restrict__ blockStart) {
+  const uint32_t j = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t n = min(*dN, maxN);
+  if (j > n) return;
+  const int64_t prev = j == 0 ? -1 : (int64_t)keys[j - 1];
+  const int64_t cur = j == n ? (int64_t)nb : (int64_t)keys[j];
+  const int64_t b0 = prev < 0 ? 0 : prev / kFirstSpan + 1;
+  const int64_t b1 = cur / kFirstSpan;
+  for (int64_t b = b0; b <= b1; ++b) blockStart[b] = j;
+}
+
 __global__ __launch_bounds__(256) void k_first(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ dN,
-                                               uint32_t nb, uint32_t* __restrict__ first,
-                                               uint32_t* __restrict__ dLive) {
+                                               uint32_t maxN, uint32_t nb, const uint32_t* __restrict__ blockStart,
+                                               uint32_t* __restrict__ first, uint32_t* __restrict__ dLive) {
   __shared__ uint32_t s_keys[kFirstStage];
-  __shared__ uint32_t s_lo, s_hi;
-  const uint32_t n = *dN;
+  const uint32_t n = min(*dN, maxN);
   const uint32_t h0 = blockIdx.x * kFirstSpan;
   const uint32_t h1 = min(h0 + kFirstSpan, nb);
-  if (threadIdx.x == 0) s_lo = lower_bound_u32(keys, 0, n, h0);
-  if (threadIdx.x == 64) s_hi = lower_bound_u32(keys, 0, n, h1);
-  __syncthreads();
-  const uint32_t lo = s_lo, hi = s_hi, m = hi - lo;
+  const uint32_t lo = min(blockStart[blockIdx.x], n);
+  const uint32_t hi = h1 == nb ? n : min(max(blockStart[blockIdx.x + 1], lo), n);
+  const uint32_t m = hi - lo;
   const bool staged = m <= kFirstStage;
   if (staged)
     for (uint32_t k = threadIdx.x; k < m; k += 256) s_keys[k] = keys[lo + k];
@@ -707,55 +721,89 @@ __device__ __forceinline__ void scan_range(const ReplicaTables& T, const float4*
 // the cell coordinates, so no pair is ever lost to rounding.
 constexpr float kFaceMargin = 1.0f / 64.0f;
 
-__global__ __launch_bounds__(256) void k_candidates(CandArgs a) {
-  const uint32_t j = blockIdx.x * 256 + threadIdx.x;
-  const uint32_t nLive = *a.dLive;
-  if (j >= nLive) return;
-  const uint32_t key = a.keys[j];
-  const int r = replica_of_bucket(a.bucketStart, a.R, key);
-  const ReplicaTables& T = a.tables[r];
-  const float4 p = a.sortedPT[j];
+// Search of one wave.  Lanes hold consecutive sorted particles, so the
+// particles of a lane's own bucket that sit in the same wave are found by
+// lane shuffles; the part of the bucket before or after the wave, and the
+// neighbouring buckets across near faces, are read from memory.
+__device__ __forceinline__ void wave_search(const ReplicaTables& T, const CandArgs& a, uint32_t j, uint32_t wbase,
+                                            uint32_t lane, bool active, uint32_t key, const float4 p, int& best) {
   const int tj = __float_as_int(p.w);
-  const uint32_t mask = (T.hasBim && tj > 0 && tj < kMaxTypes) ? T.partnerMask[tj] : 0u;
-  int best = -1;
-  if (mask) {
-    const float maxB2 = T.maxBindR2[tj];
-    const float pc[3] = {p.x, p.y, p.z};
-    int lo[3], hi[3];
+  const uint32_t mask = (active && T.hasBim && tj > 0 && tj < kMaxTypes) ? T.partnerMask[tj] : 0u;
+  const float maxB2 = mask ? T.maxBindR2[tj] : 0.0f;
+  float bestD2 = 3.4e38f;
+  // own bucket, inside the wave
+  const uint32_t nLive = *a.dLive;
+  for (int o = 1; o < 64; ++o) {
+    const int up = (int)lane + o, dn = (int)lane - o;
+    const uint32_t kUp = __shfl(key, up & 63, 64);
+    const uint32_t kDn = __shfl(key, dn & 63, 64);
+    const bool hitUp = active && up < 64 && kUp == key;
+    const bool hitDn = active && dn >= 0 && kDn == key;
+    if (!__any(hitUp || hitDn)) break;  // runs are contiguous: no lane has a partner further out
+    const float4 qUp = make_float4(__shfl(p.x, up & 63, 64), __shfl(p.y, up & 63, 64), __shfl(p.z, up & 63, 64),
+                                   __shfl(p.w, up & 63, 64));
+    const float4 qDn = make_float4(__shfl(p.x, dn & 63, 64), __shfl(p.y, dn & 63, 64), __shfl(p.z, dn & 63, 64),
+                                   __shfl(p.w, dn & 63, 64));
+    if (mask && hitUp) consider(T, qUp, wbase + (uint32_t)up, j, p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
+    if (mask && hitDn) consider(T, qDn, wbase + (uint32_t)dn, j, p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
+  }
+  if (!mask) return;
+  const uint32_t* __restrict__ first = a.first;
+  // own bucket, outside the wave
+  const uint32_t key0 = __shfl(key, 0, 64), key63 = __shfl(key, 63, 64);
+  if (key == key0 && wbase > 0 && a.keys[wbase - 1] == key)
+    scan_range(T, a.sortedPT, first[key], wbase, j, p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
+  if (key == key63 && wbase + 64 < nLive && a.keys[wbase + 64] == key)
+    scan_range(T, a.sortedPT, wbase + 64, first[key + 1], j, p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
+  // neighbouring buckets across the faces this particle is near
+  const float pc[3] = {p.x, p.y, p.z};
+  int lo[3], hi[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const float q = (pc[d] - T.origin[d]) / T.cellSize[d];
-      const float f = q - floorf(q);
-      const float rb = sqrtf(maxB2) / T.cellSize[d] + kFaceMargin;
-      lo[d] = f < rb ? 1 : 0;
-      hi[d] = f > 1.0f - rb ? 1 : 0;
-    }
-    float bestD2 = 3.4e38f;
-    const uint32_t local = key - T.bucketOff;
-    const uint32_t bx = T.gbits[0], by = T.gbits[1];
-    const uint32_t mx = (1u << bx) - 1u, my = (1u << by) - 1u, mz = (1u << T.gbits[2]) - 1u;
-    const uint32_t gx = local & mx, gy = (local >> bx) & my, gz = local >> (bx + by);
-    const uint32_t* __restrict__ first = a.first;
-    const bool xRun = gx >= (uint32_t)lo[0] && gx + (uint32_t)hi[0] <= mx;  // x range without wrap
-    for (int dz = -lo[2]; dz <= hi[2]; ++dz) {
-      const uint32_t zz = (gz + (uint32_t)dz) & mz;
-      for (int dy = -lo[1]; dy <= hi[1]; ++dy) {
-        const uint32_t yy = (gy + (uint32_t)dy) & my;
-        const uint32_t row = T.bucketOff + (zz << (bx + by)) + (yy << bx);
-        if (xRun) {
-          scan_range(T, a.sortedPT, first[row + gx - (uint32_t)lo[0]], first[row + gx + (uint32_t)hi[0] + 1u], j,
-                     p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
-        } else {
-          for (int dx = -lo[0]; dx <= hi[0]; ++dx) {
-            const uint32_t xx = (gx + (uint32_t)dx) & mx;
-            scan_range(T, a.sortedPT, first[row + xx], first[row + xx + 1u], j, p.x, p.y, p.z, tj, mask, maxB2,
-                       best, bestD2);
-          }
-        }
+  for (int d = 0; d < 3; ++d) {
+    const float q = (pc[d] - T.origin[d]) / T.cellSize[d];
+    const float f = q - floorf(q);
+    const float rb = sqrtf(maxB2) / T.cellSize[d] + kFaceMargin;
+    lo[d] = f < rb ? 1 : 0;
+    hi[d] = f > 1.0f - rb ? 1 : 0;
+  }
+  if (!(lo[0] | hi[0] | lo[1] | hi[1] | lo[2] | hi[2])) return;
+  const uint32_t local = key - T.bucketOff;
+  const uint32_t bx = T.gbits[0], by = T.gbits[1];
+  const uint32_t mx = (1u << bx) - 1u, my = (1u << by) - 1u, mz = (1u << T.gbits[2]) - 1u;
+  const uint32_t gx = local & mx, gy = (local >> bx) & my, gz = local >> (bx + by);
+  for (int dz = -lo[2]; dz <= hi[2]; ++dz) {
+    const uint32_t zz = (gz + (uint32_t)dz) & mz;
+    for (int dy = -lo[1]; dy <= hi[1]; ++dy) {
+      const uint32_t yy = (gy + (uint32_t)dy) & my;
+      const uint32_t row = T.bucketOff + (zz << (bx + by)) + (yy << bx);
+      for (int dx = -lo[0]; dx <= hi[0]; ++dx) {
+        if (dx == 0 && dy == 0 && dz == 0) continue;  // own bucket done above
+        const uint32_t xx = (gx + (uint32_t)dx) & mx;
+        scan_range(T, a.sortedPT, first[row + xx], first[row + xx + 1u], j, p.x, p.y, p.z, tj, mask, maxB2, best,
+                   bestD2);
       }
     }
   }
-  a.cand[j] = best;
+}
+
+__global__ __launch_bounds__(256) void k_candidates(CandArgs a) {
+  const uint32_t j = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wbase = j - lane;
+  const uint32_t nLive = *a.dLive;
+  if (wbase >= nLive) return;  // the whole wave is past the live particles
+  const bool active = j < nLive;
+  // inactive lanes get a key no live particle has
+  const uint32_t key = active ? a.keys[j] : 0xFFFFFFFFu;
+  const float4 p = active ? a.sortedPT[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const int r = active ? replica_of_bucket(a.bucketStart, a.R, key) : 0;
+  const int rw = __builtin_amdgcn_readfirstlane(r);
+  int best = -1;
+  if (!__any(active && r != rw))
+    wave_search(a.tables[rw], a, j, wbase, lane, active, key, p, best);  // one replica: table loads are scalar
+  else
+    wave_search(a.tables[r], a, j, wbase, lane, active, key, p, best);
+  if (active) a.cand[j] = best;
 }
 
 // ---------------------------------------------------------------------------
@@ -949,13 +997,14 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
   if (tile * kCompactTile >= n) return;
   const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
   const uint64_t lt = lanemask_lt();
-  constexpr int kItems = kCompactTile / 256;  // 4 chunks of 64 per wave
+  constexpr int kItems = kCompactTile / 256;  // chunks of 64 per wave
+  constexpr uint32_t kWaveSpan = kCompactTile / 4;  // consecutive elements per wave
   bool al[kItems], bi[kItems];
   uint32_t ra[kItems], rb[kItems];
   uint32_t ca = 0, cb = 0;
 #pragma unroll
   for (int it = 0; it < kItems; ++it) {
-    const uint32_t e = tile * kCompactTile + w * 256 + it * 64 + lane;
+    const uint32_t e = tile * kCompactTile + w * kWaveSpan + it * 64 + lane;
     const bool v = e < n;
     al[it] = v && a.type[e] > 0;
     bi[it] = v && a.birthType[e] > 0;
@@ -976,18 +1025,24 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
     pa += s_wa[k];
     pb += s_wb[k];
   }
-#pragma unroll
-  for (int it = 0; it < kItems; ++it) {
-    const uint32_t e = tile * kCompactTile + w * 256 + it * 64 + lane;
-    if (!(al[it] || bi[it])) continue;
-    // replica of element e: last r with repOff[r] <= e
+  // replica of this thread's first element (last r with repOff[r] <= e);
+  // later elements of the thread only move forward through the replicas
+  int r = 0;
+  {
+    const uint32_t e0 = tile * kCompactTile + w * kWaveSpan + lane;
     int lo = 0, hi = a.R;
     while (hi - lo > 1) {
       const int mid = (lo + hi) >> 1;
-      if (a.repOff[mid] <= e) lo = mid;
+      if (a.repOff[mid] <= e0) lo = mid;
       else hi = mid;
     }
-    const int r = lo;
+    r = lo;
+  }
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    const uint32_t e = tile * kCompactTile + w * kWaveSpan + it * 64 + lane;
+    if (!(al[it] || bi[it])) continue;
+    while (r + 1 < a.R && a.repOff[r + 1] <= e) ++r;
     if (al[it]) {
       const uint32_t dst = a.newOff[r] + (pa + ra[it] - a.aliveBase[r]);
       a.posOut[dst] = a.pos[e];
@@ -1232,9 +1287,15 @@ hipError_t launch_sort_3k(SortBuffers& b, const uint32_t* dN, uint32_t maxN, uin
   return hipGetLastError();
 }
 
-hipError_t launch_first(const uint32_t* keys, const uint32_t* dN, uint32_t nb, uint32_t* first, uint32_t* dLive,
-                        hipStream_t stream) {
-  hipLaunchKernelGGL(k_first, dim3(cdiv(nb, kFirstSpan)), dim3(256), 0, stream, keys, dN, nb, first, dLive);
+size_t first_words(uint32_t nb) { return (size_t)nb + 1 + cdiv(nb, kFirstSpan) + 1; }
+
+hipError_t launch_first(const uint32_t* keys, const uint32_t* dN, uint32_t maxN, uint32_t nb, uint32_t* first,
+                        uint32_t* dLive, hipStream_t stream) {
+  uint32_t* blockStart = first + nb + 1;  // cdiv(nb, kFirstSpan) + 1 words after first[0..nb]
+  hipLaunchKernelGGL(k_first_blocks, dim3(cdiv((size_t)maxN + 1, 256)), dim3(256), 0, stream, keys, dN, maxN, nb,
+                     blockStart);
+  hipLaunchKernelGGL(k_first, dim3(cdiv(nb, kFirstSpan)), dim3(256), 0, stream, keys, dN, maxN, nb,
+                     (const uint32_t*)blockStart, first, dLive);
   return hipGetLastError();
 }
 

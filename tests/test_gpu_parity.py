@@ -325,7 +325,7 @@ def test_autotune_picks_a_valid_fastest_config(smd, gpu):
     assert len(table) >= 2
     # the tuned grid still runs and conserves A + C and B + C
     e = smd.Engine(0, best[1])
-    e.add_replica(autotune.apply(spec, best[0]), pos, types)
+    e.add_replica(autotune.apply(spec, best[0], best[2]), pos, types)
     e.finalize()
     e.step(20)
     c = e.species_counts(0)
