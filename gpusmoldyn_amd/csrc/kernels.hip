@@ -747,10 +747,11 @@ __device__ __forceinline__ void wave_search(const ReplicaTables& T, const CandAr
     if (mask && hitUp) consider(T, qUp, wbase + (uint32_t)up, j, p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
     if (mask && hitDn) consider(T, qDn, wbase + (uint32_t)dn, j, p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
   }
+  // the wave's first and last keys, read while every lane is still active
+  const uint32_t key0 = __shfl(key, 0, 64), key63 = __shfl(key, 63, 64);
   if (!mask) return;
   const uint32_t* __restrict__ first = a.first;
   // own bucket, outside the wave
-  const uint32_t key0 = __shfl(key, 0, 64), key63 = __shfl(key, 63, 64);
   if (key == key0 && wbase > 0 && a.keys[wbase - 1] == key)
     scan_range(T, a.sortedPT, first[key], wbase, j, p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
   if (key == key63 && wbase + 64 < nLive && a.keys[wbase + 64] == key)
