@@ -623,6 +623,7 @@ __global__ __launch_bounds__(256) void k_first(const uint32_t* __restrict__ keys
                                                uint32_t maxN, uint32_t nb, const uint32_t* __restrict__ blockStart,
                                                uint32_t* __restrict__ first, uint32_t* __restrict__ dLive) {
   __shared__ uint32_t s_keys[kFirstStage];
+  __shared__ uint32_t s_out[kFirstSpan];
   const uint32_t n = min(*dN, maxN);
   const uint32_t h0 = blockIdx.x * kFirstSpan;
   const uint32_t h1 = min(h0 + kFirstSpan, nb);
@@ -633,19 +634,36 @@ __global__ __launch_bounds__(256) void k_first(const uint32_t* __restrict__ keys
   if (staged)
     for (uint32_t k = threadIdx.x; k < m; k += 256) s_keys[k] = keys[lo + k];
   __syncthreads();
-  for (uint32_t h = h0 + threadIdx.x; h < h1; h += 256) {
-    uint32_t f;
+  // each thread: a binary search for the first of its kFirstSpan / 256
+  // consecutive buckets, then a forward walk over the staged keys
+  constexpr uint32_t kPer = kFirstSpan / 256;
+  const uint32_t c0 = h0 + threadIdx.x * kPer;
+  if (c0 < h1) {
+    const uint32_t c1 = min(c0 + kPer, h1);
     if (staged) {
-      uint32_t a = 0, b = m;
-      while (a < b) {
-        const uint32_t mid = (a + b) >> 1;
-        if (s_keys[mid] < h) a = mid + 1;
-        else b = mid;
+      uint32_t lo2 = 0, hi2 = m;
+      while (lo2 < hi2) {
+        const uint32_t mid = (lo2 + hi2) >> 1;
+        if (s_keys[mid] < c0) lo2 = mid + 1;
+        else hi2 = mid;
       }
-      f = lo + a;
+      uint32_t ptr = lo2;
+      for (uint32_t h = c0; h < c1; ++h) {
+        while (ptr < m && s_keys[ptr] < h) ++ptr;
+        s_out[h - h0] = lo + ptr;
+      }
     } else {
-      f = lower_bound_u32(keys, lo, hi, h);
+      uint32_t ptr = lower_bound_u32(keys, lo, hi, c0);
+      for (uint32_t h = c0; h < c1; ++h) {
+        while (ptr < hi && keys[ptr] < h) ++ptr;
+        s_out[h - h0] = ptr;
+      }
     }
+  }
+  __syncthreads();
+  // coalesced stores
+  for (uint32_t h = h0 + threadIdx.x; h < h1; h += 256) {
+    const uint32_t f = s_out[h - h0];
     first[h] = f;
     if (h == nb - 1) *dLive = f;
   }
@@ -959,26 +977,42 @@ __global__ __launch_bounds__(1024) void k_finalize(FinalizeArgs a) {
     *a.dNnext = cN;
     *a.particleSteps += n;
   }
-  // compaction tiles
+  // compaction tiles, four consecutive tiles per thread
+  constexpr uint32_t kTPT = 4;
   const uint32_t nTiles = (n + kCompactTile - 1) / kCompactTile;
   uint32_t carA = 0, carB = 0;
-  for (uint32_t b = 0; b < nTiles; b += 1024) {
-    const uint32_t e = b + threadIdx.x;
-    uint32_t va = 0, vb = 0;
-    if (e < nTiles) {
-      va = a.tileAlive[e];
-      vb = a.tileBirth[e];
-      a.tileAlive[e] = 0;
-      a.tileBirth[e] = 0;
+  for (uint32_t b = 0; b < nTiles; b += 1024 * kTPT) {
+    const uint32_t e0 = b + threadIdx.x * kTPT;
+    uint32_t va[kTPT], vb[kTPT], sa = 0, sbv = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kTPT; ++k) {
+      const uint32_t e = e0 + k;
+      va[k] = 0;
+      vb[k] = 0;
+      if (e < nTiles) {
+        va[k] = a.tileAlive[e];
+        vb[k] = a.tileBirth[e];
+        a.tileAlive[e] = 0;
+        a.tileBirth[e] = 0;
+      }
+      sa += va[k];
+      sbv += vb[k];
     }
-    const uint32_t exA = block_exscan1024(va, wsum, &tot);
+    const uint32_t exA = block_exscan1024(sa, wsum, &tot);
     const uint32_t tA = tot;
     __syncthreads();
-    const uint32_t exB = block_exscan1024(vb, wsum, &tot);
+    const uint32_t exB = block_exscan1024(sbv, wsum, &tot);
     const uint32_t tB = tot;
-    if (e < nTiles) {
-      a.tileAlivePre[e] = carA + exA;
-      a.tileBirthPre[e] = carB + exB;
+    uint32_t ra = carA + exA, rb = carB + exB;
+#pragma unroll
+    for (uint32_t k = 0; k < kTPT; ++k) {
+      const uint32_t e = e0 + k;
+      if (e < nTiles) {
+        a.tileAlivePre[e] = ra;
+        a.tileBirthPre[e] = rb;
+      }
+      ra += va[k];
+      rb += vb[k];
     }
     carA += tA;
     carB += tB;

@@ -15,7 +15,7 @@ constexpr int kMtMM = 9;         // middle word offset (MersenneTwister.cuh:29)
 constexpr int kMtWords = 20;     // 19 state words + iState, stored word-major (SoA)
 constexpr int kMtParamSets = 32768;  // records in MersenneTwister.dat (MersenneTwister.h:31)
 constexpr int kGaussDim = 4096;  // particles_kernel.cuh:43
-constexpr int kCompactTile = 4096;   // elements per compaction tile (4 waves x 16 chunks of 64)
+constexpr int kCompactTile = 1024;   // elements per compaction tile (4 waves x 4 chunks of 64)
 
 // Boundary behaviour of one face of the simulation box.
 enum Boundary : int32_t {
