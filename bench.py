@@ -76,7 +76,7 @@ def build(cfg):
     return reps, steps, desc
 
 
-def make_engine(reps, radix_bits=8):
+def make_engine(reps, radix_bits=0):
     e = smd.Engine(0, radix_bits)
     for spec, pos, types in reps:
         e.add_replica(spec, pos, types)
@@ -100,7 +100,8 @@ def roofline(e, reps, nprof=5):
     lay = e.layout()
     n = int(e.counts().sum())
     S = lay["total_streams"]
-    passes = (lay["key_bits"] + 7) // 8
+    rb = e.radix_bits()
+    passes = (lay["key_bits"] + rb - 1) // rb
     rows = {}
     for k, us in prof.items():
         b = stage_bytes(k, n, S, lay["total_buckets"], passes)
@@ -176,7 +177,7 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="c2")
-    ap.add_argument("--radix-bits", type=int, default=8)
+    ap.add_argument("--radix-bits", type=int, default=0, help="sort digit width (4, 8, 11; 0 = automatic)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="c4: skip the one-replica-per-launch comparison")
     ap.add_argument("--no-graphs", action="store_true", help="one launch sequence per step instead of graphs")

@@ -99,6 +99,7 @@ def lib():
         L.smd_engine_set_graphs.argtypes = [vp, ctypes.c_int]
         L.smd_engine_set_sort.argtypes = [vp, ctypes.c_int]
         L.smd_engine_num_replicas.argtypes = [vp]
+        L.smd_engine_radix_bits.argtypes = [vp]
         L.smd_engine_num_replicas.restype = ctypes.c_uint
         L.smd_engine_get.argtypes = [vp, ctypes.c_int, f32p, i32p, ctypes.c_uint, ctypes.POINTER(ctypes.c_uint)]
         L.smd_engine_species_counts.argtypes = [vp, ctypes.c_int, u32p]
@@ -225,7 +226,7 @@ class Engine:
     """An ensemble of independent replicas stepped together on one GPU (a
     plain simulation is an ensemble of one)."""
 
-    def __init__(self, device=0, radix_bits=8):
+    def __init__(self, device=0, radix_bits=0):
         L = lib()
         h = L.smd_engine_create(device, radix_bits)
         if not h:
@@ -328,6 +329,9 @@ class Engine:
         out = (RxnDerived * MAX_RXN)()
         _check(lib().smd_engine_rxn(self._h, r, out), "smd_engine_rxn")
         return [out[k] for k in range(self.specs[r].num_reactions)]
+
+    def radix_bits(self):
+        return lib().smd_engine_radix_bits(self._h)
 
     def layout(self):
         v = [ctypes.c_uint(0) for _ in range(4)]

@@ -168,12 +168,12 @@ smd_engine* smd_engine_create(int device, int radix_bits) {
     set_error("device index out of range");
     return nullptr;
   }
-  if (radix_bits != 0 && radix_bits != 4 && radix_bits != 8) {
-    set_error("radix_bits must be 0, 4 or 8");
+  if (radix_bits != 0 && radix_bits != 4 && radix_bits != 8 && radix_bits != 11) {
+    set_error("radix_bits must be 0, 4, 8 or 11");
     return nullptr;
   }
   smd_engine* e = new smd_engine;
-  e->impl = new Engine(device, radix_bits == 0 ? 8 : radix_bits);
+  e->impl = new Engine(device, radix_bits);
   return e;
 }
 
@@ -247,6 +247,8 @@ smd_status smd_engine_particle_steps(smd_engine* e, unsigned long long* out) {
   if (!e || !out) return SMD_ERR_INVALID;
   return e->impl->particle_steps(out);
 }
+
+int smd_engine_radix_bits(const smd_engine* e) { return e ? e->impl->radixBits : 0; }
 
 unsigned smd_engine_num_replicas(const smd_engine* e) { return e ? (unsigned)e->impl->tabs.size() : 0u; }
 
@@ -348,7 +350,7 @@ smd_status smd_mt_generate(int device, const uint32_t* seeds, unsigned n_streams
 smd_status smd_sort_pairs(int device, uint32_t* keys, uint32_t* values, unsigned n, unsigned key_bits,
                           int radix_bits) {
   if (n == 0) return SMD_OK;
-  if (!keys || !values || key_bits == 0 || key_bits > 32 || (radix_bits != 4 && radix_bits != 8)) {
+  if (!keys || !values || key_bits == 0 || key_bits > 32 || (radix_bits != 4 && radix_bits != 8 && radix_bits != 11)) {
     set_error("invalid sort arguments");
     return SMD_ERR_INVALID;
   }

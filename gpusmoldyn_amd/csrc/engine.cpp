@@ -39,7 +39,7 @@ static void dfree(T*& p) {
 
 static inline uint32_t round64(uint32_t x) { return (x + 63u) & ~63u; }
 
-Engine::Engine(int dev, int rbits) : device(dev), radixBits(rbits == 4 ? 4 : 8) {}
+Engine::Engine(int dev, int rbits) : device(dev), radixBits(rbits == 4 || rbits == 8 || rbits == 11 ? rbits : 0) {}
 
 Engine::~Engine() {
   drop_graph();
@@ -194,6 +194,11 @@ smd_status Engine::finalize(uint32_t capacity) {
   deadKey = (uint32_t)buckets;
   keyBits = 0;
   while ((1ull << keyBits) <= deadKey) ++keyBits;
+  if (radixBits == 0) {
+    // automatic digit width: 11-bit digits when they save a pass over 8-bit ones
+    const uint32_t p8 = (keyBits + 7) / 8, p11 = (keyBits + 10) / 11;
+    radixBits = p11 < p8 ? 11 : 8;
+  }
   nb = deadKey + 1;
   Stot = (uint32_t)streams;
   uint64_t c = capacity ? capacity : std::max<uint64_t>(anySplit ? 8 * total : total, 1024);

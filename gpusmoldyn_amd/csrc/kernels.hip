@@ -1273,8 +1273,26 @@ static hipError_t launch_sort_onesweep(SortBuffers& b, const uint32_t* dN, uint3
 
 hipError_t launch_sort(SortBuffers& b, const uint32_t* dN, uint32_t maxN, uint32_t keyBits, int radixBits,
                        bool identityVals, hipStream_t stream) {
-  if (b.algo == 1) return launch_sort_3k(b, dN, maxN, keyBits, radixBits, identityVals, stream);
+  // the single sweep keeps every digit's histogram in LDS at once: 4- and 8-bit digits only
+  if (b.algo == 1 || (radixBits != 4 && radixBits != 8))
+    return launch_sort_3k(b, dN, maxN, keyBits, radixBits, identityVals, stream);
   return launch_sort_onesweep(b, dN, maxN, keyBits, radixBits, identityVals, stream);
+}
+
+// One digit pass of the count / scan / scatter sort.
+template <int BITS>
+static hipError_t sort3k_pass(SortBuffers& b, const uint32_t* dN, uint32_t maxTiles, uint32_t stride,
+                              uint32_t* totals, int shift, bool idv, bool gather, hipStream_t stream) {
+  hipLaunchKernelGGL(k_sort_up<BITS>, dim3(maxTiles), dim3(256), 0, stream, b.keys, dN, b.counts, stride, shift);
+  hipLaunchKernelGGL(k_sort_rowscan<BITS>, dim3(1 << BITS), dim3(256), 0, stream, b.counts, totals, dN, stride);
+  if (gather)
+    hipLaunchKernelGGL((k_sort_down<BITS, true>), dim3(maxTiles), dim3(256), 0, stream, b.keys, b.vals, b.keysAlt,
+                       b.valsAlt, b.counts, totals, dN, stride, shift, idv ? 1 : 0, b.gatherSrc, b.gatherDst,
+                       b.gatherDeadKey);
+  else
+    hipLaunchKernelGGL((k_sort_down<BITS, false>), dim3(maxTiles), dim3(256), 0, stream, b.keys, b.vals, b.keysAlt,
+                       b.valsAlt, b.counts, totals, dN, stride, shift, idv ? 1 : 0, nullptr, nullptr, 0u);
+  return hipGetLastError();
 }
 
 hipError_t launch_sort_3k(SortBuffers& b, const uint32_t* dN, uint32_t maxN, uint32_t keyBits, int radixBits,
@@ -1289,29 +1307,11 @@ hipError_t launch_sort_3k(SortBuffers& b, const uint32_t* dN, uint32_t maxN, uin
     const int shift = p * radixBits;
     const bool idv = identityVals && p == 0;
     const bool gather = b.gatherDst != nullptr && p == passes - 1;
-    if (radixBits == 8) {
-      hipLaunchKernelGGL(k_sort_up<8>, dim3(maxTiles), dim3(256), 0, stream, b.keys, dN, b.counts, stride, shift);
-      hipLaunchKernelGGL(k_sort_rowscan<8>, dim3(radix), dim3(256), 0, stream, b.counts, totals, dN, stride);
-      if (gather)
-        hipLaunchKernelGGL((k_sort_down<8, true>), dim3(maxTiles), dim3(256), 0, stream, b.keys, b.vals, b.keysAlt,
-                           b.valsAlt, b.counts, totals, dN, stride, shift, idv ? 1 : 0, b.gatherSrc, b.gatherDst,
-                           b.gatherDeadKey);
-      else
-        hipLaunchKernelGGL((k_sort_down<8, false>), dim3(maxTiles), dim3(256), 0, stream, b.keys, b.vals, b.keysAlt,
-                           b.valsAlt, b.counts, totals, dN, stride, shift, idv ? 1 : 0, nullptr, nullptr, 0u);
-    } else if (radixBits == 4) {
-      hipLaunchKernelGGL(k_sort_up<4>, dim3(maxTiles), dim3(256), 0, stream, b.keys, dN, b.counts, stride, shift);
-      hipLaunchKernelGGL(k_sort_rowscan<4>, dim3(radix), dim3(256), 0, stream, b.counts, totals, dN, stride);
-      if (gather)
-        hipLaunchKernelGGL((k_sort_down<4, true>), dim3(maxTiles), dim3(256), 0, stream, b.keys, b.vals, b.keysAlt,
-                           b.valsAlt, b.counts, totals, dN, stride, shift, idv ? 1 : 0, b.gatherSrc, b.gatherDst,
-                           b.gatherDeadKey);
-      else
-        hipLaunchKernelGGL((k_sort_down<4, false>), dim3(maxTiles), dim3(256), 0, stream, b.keys, b.vals, b.keysAlt,
-                           b.valsAlt, b.counts, totals, dN, stride, shift, idv ? 1 : 0, nullptr, nullptr, 0u);
-    } else {
-      return hipErrorInvalidValue;
-    }
+    hipError_t e = hipErrorInvalidValue;
+    if (radixBits == 8) e = sort3k_pass<8>(b, dN, maxTiles, stride, totals, shift, idv, gather, stream);
+    else if (radixBits == 4) e = sort3k_pass<4>(b, dN, maxTiles, stride, totals, shift, idv, gather, stream);
+    else if (radixBits == 11) e = sort3k_pass<11>(b, dN, maxTiles, stride, totals, shift, idv, gather, stream);
+    if (e != hipSuccess) return e;
     uint32_t* tk = b.keys;
     b.keys = b.keysAlt;
     b.keysAlt = tk;

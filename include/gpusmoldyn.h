@@ -157,8 +157,9 @@ smd_status smd_mt_params(unsigned index, uint32_t* abc);
 /* ---- simulation engine ---- */
 
 smd_status smd_device_count(int* n);
-/* Create an engine on GPU `device`.  radix_bits: 4 or 8 (sort digit width),
- * 0 = default (8). */
+/* Create an engine on GPU `device`.  radix_bits: sort digit width 4, 8 or
+ * 11; 0 = automatic (11 when it saves a pass over 8, else 8).  The sorted
+ * order, hence every result, is the same for any width. */
 smd_engine* smd_engine_create(int device, int radix_bits);
 void smd_engine_destroy(smd_engine* e);
 /* Add a replica with n initial particles (pos: n float4, types: n ints).
@@ -181,6 +182,8 @@ smd_status smd_engine_sync(smd_engine* e);
 /* Current particle count of each replica (synchronises). */
 smd_status smd_engine_counts(smd_engine* e, unsigned* counts);
 unsigned smd_engine_num_replicas(const smd_engine* e);
+/* Sort digit width in use (fixed at finalize when created with 0). */
+int smd_engine_radix_bits(const smd_engine* e);
 /* Sum over all steps so far of the particles entering each step (all
  * replicas): the numerator of particle-steps/s. */
 smd_status smd_engine_particle_steps(smd_engine* e, unsigned long long* out);
@@ -232,7 +235,8 @@ smd_status smd_mt_generate(int device, const uint32_t* seeds, unsigned n_streams
                            uint32_t* out);
 
 /* Stable sort of (key, value) pairs on the GPU by the low key_bits bits
- * (RadixSort::sort, radixsort.h:168-185).  Host arrays, n elements. */
+ * (RadixSort::sort, radixsort.h:168-185).  Host arrays, n elements;
+ * radix_bits 4, 8 or 11. */
 smd_status smd_sort_pairs(int device, uint32_t* keys, uint32_t* values, unsigned n, unsigned key_bits,
                           int radix_bits);
 

@@ -36,7 +36,7 @@ def test_mt_matches_reference_dcmt(smd, gpu):
         assert np.array_equal(got[row], outs[row])
 
 
-@pytest.mark.parametrize("radix_bits", [4, 8])
+@pytest.mark.parametrize("radix_bits", [4, 8, 11])
 @pytest.mark.parametrize("n,bits", [(0, 8), (1, 8), (4095, 12), (4096, 12), (4097, 20), (100003, 24), (1 << 20, 21),
                                     (3000000, 25)])
 def test_sort_pairs_stable(smd, gpu, n, bits, radix_bits):
@@ -142,7 +142,7 @@ WORKLOADS = {
 }
 
 
-def _engine(smd, spec, pos, types, radix_bits=8):
+def _engine(smd, spec, pos, types, radix_bits=0):
     e = smd.Engine(0, radix_bits)
     e.add_replica(spec, pos, types)
     e.finalize()
@@ -230,16 +230,18 @@ def test_ensemble_equals_alone(smd, gpu):
     e.close()
 
 
-def test_radix4_equals_radix8(smd, gpu):
+def test_digit_widths_agree(smd, gpu):
     spec, pos, types = w_periodic_reversible()
-    banner("radix 4 vs 8")
-    a = _engine(smd, spec, pos, types, radix_bits=4)
-    b = _engine(smd, spec, pos, types, radix_bits=8)
-    a.step(15)
-    b.step(15)
-    ap, at = a.get(0)
-    bp, bt = b.get(0)
-    assert np.array_equal(at, bt) and np.array_equal(ap.view(np.uint32), bp.view(np.uint32))
+    banner("radix 4 vs 8 vs 11")
+    res = []
+    for rb in (4, 8, 11):
+        e = _engine(smd, spec, pos, types, radix_bits=rb)
+        assert e.radix_bits() == rb
+        e.step(15)
+        res.append(e.get(0))
+        e.close()
+    for p, t in res[1:]:
+        assert np.array_equal(res[0][1], t) and np.array_equal(res[0][0].view(np.uint32), p.view(np.uint32))
 
 
 def test_checkpoint_resume(smd, gpu):
