@@ -194,11 +194,10 @@ smd_status Engine::finalize(uint32_t capacity) {
   deadKey = (uint32_t)buckets;
   keyBits = 0;
   while ((1ull << keyBits) <= deadKey) ++keyBits;
-  if (radixBits == 0) {
-    // automatic digit width: 11-bit digits when they save a pass over 8-bit ones
-    const uint32_t p8 = (keyBits + 7) / 8, p11 = (keyBits + 10) / 11;
-    radixBits = p11 < p8 ? 11 : 8;
-  }
+  // automatic digit width: 8 bits (11-bit digits save a pass for 17-22-bit
+  // keys but each pass costs more than the pass saved: 80 KB of LDS per
+  // workgroup and 2048 scan rows)
+  if (radixBits == 0) radixBits = 8;
   nb = deadKey + 1;
   Stot = (uint32_t)streams;
   uint64_t c = capacity ? capacity : std::max<uint64_t>(anySplit ? 8 * total : total, 1024);

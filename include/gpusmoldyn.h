@@ -158,8 +158,8 @@ smd_status smd_mt_params(unsigned index, uint32_t* abc);
 
 smd_status smd_device_count(int* n);
 /* Create an engine on GPU `device`.  radix_bits: sort digit width 4, 8 or
- * 11; 0 = automatic (11 when it saves a pass over 8, else 8).  The sorted
- * order, hence every result, is the same for any width. */
+ * 11; 0 = automatic (8).  The sorted order, hence every result, is the same
+ * for any width. */
 smd_engine* smd_engine_create(int device, int radix_bits);
 void smd_engine_destroy(smd_engine* e);
 /* Add a replica with n initial particles (pos: n float4, types: n ints).
