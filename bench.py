@@ -95,7 +95,28 @@ def timed(e, steps):
     return ps1 - ps0, t1 - t0
 
 
-def roofline(e, reps, nprof=5):
+# stages that are one kernel each, and that kernel's name in the profiles
+SINGLE_KERNEL = {"producer": "k_producer", "candidates": "k_candidates", "resolve": "k_resolve",
+                 "scatter": "k_scatter"}
+PROFILE = {"c2": "step_kernels_c2_1M.md", "c3": "step_kernels_c3_4M.md", "c4": "step_kernels_c4_9M.md"}
+
+
+def profiled_traffic(cfg, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
+    (FETCH_SIZE and WRITE_SIZE scaled by a plain copy of known size), with
+    that summary's average duration; (None, None) without a summary."""
+    name = PROFILE.get(cfg)
+    path = os.path.join(ROOT, "profiles", name) if name else None
+    if not path or not os.path.exists(path):
+        return None, None, None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 7 and cells[0] == kernel and cells[6] not in ("None", "traffic_bytes"):
+            return int(cells[6]), float(cells[2]), "profiles/" + name
+    return None, None, None
+
+
+def roofline(e, reps, cfg, nprof=5):
     prof = e.profile(nprof)
     lay = e.layout()
     n = int(e.counts().sum())
@@ -106,13 +127,16 @@ def roofline(e, reps, nprof=5):
     for k, us in prof.items():
         b = stage_bytes(k, n, S, lay["total_buckets"], passes)
         rows[k] = dict(us=us, bytes=b, gbs=(b / (us * 1e-6) / 1e9) if us > 0 else 0.0)
-    dom = max(rows, key=lambda k: rows[k]["us"])
+    # the dominant kernel: the longest stage that is a single kernel
+    dom = max((k for k in rows if k in SINGLE_KERNEL), key=lambda k: rows[k]["us"])
     step_us = sum(r["us"] for r in rows.values())
     step_bytes = sum(r["bytes"] for r in rows.values())
     ach = rows[dom]["gbs"]
-    rl = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-          "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-          "bytes_per_launch": rows[dom]["bytes"], "avg_us": round(rows[dom]["us"], 2)}
+    traffic, prof_us, prof_file = profiled_traffic(cfg, SINGLE_KERNEL[dom])
+    rl = {"bound": "hbm", "kernel": SINGLE_KERNEL[dom], "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+          "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
+          "bytes_per_launch": rows[dom]["bytes"], "avg_us": round(rows[dom]["us"], 2),
+          "profile": prof_file, "profile_avg_us": prof_us}
     step = {"us": round(step_us, 2), "bytes": step_bytes,
             "gbs": round(step_bytes / (step_us * 1e-6) / 1e9, 1) if step_us > 0 else 0.0,
             "stages_us": {k: round(v["us"], 2) for k, v in rows.items()}}
@@ -207,7 +231,7 @@ def main():
         e.set_sort(args.sort)
     e.step(args.warmup)
     ps, secs = timed(e, steps)
-    rl, step = roofline(e, reps)
+    rl, step = roofline(e, reps, args.config)
     out = {
         "metric": METRIC,
         "value": ps / secs,
