@@ -8,7 +8,7 @@ each spec's `seed`.
   c2         1,048,576 particles, A + B <-> C and C -> 0, periodic 100 um cube
   c3         4,194,304 particles in 64 Gaussian blobs, A + B <-> C, reflective
              sphere r = 60 um, absorbing plane z = 0, 200 um cube
-  (c2 and c3 enter in hash-bucket order, as a lattice initialisation would)
+  (c2, c3 and c4 enter in hash-bucket order, as a lattice initialisation would)
   c4         512 replicas of c1's network, n log-uniform in [2048, 65536],
              kf log-uniform in [1e-4, 1e-2]
 """
@@ -180,7 +180,7 @@ def streams_for(n, per=8):
     return max(64, int(math.ceil(n / per)))
 
 
-def c4(n_rep=512, nmin=2048, nmax=65536, kmin=1e-4, kmax=1e-2, per_stream=8):
+def c4(n_rep=512, nmin=2048, nmax=65536, kmin=1e-4, kmax=1e-2, per_stream=8, ordered=True):
     """List of (spec, pos, types) for the batched ensemble."""
     rng = np.random.default_rng(SEED)
     ns = np.exp(rng.uniform(math.log(nmin), math.log(nmax), size=n_rep)).astype(np.int64)
@@ -191,6 +191,8 @@ def c4(n_rep=512, nmin=2048, nmax=65536, kmin=1e-4, kmax=1e-2, per_stream=8):
         nA = n // 2
         spec, pos, types = c1(nA=nA, nB=n - nA, kf=float(ks[i]), n_streams=streams_for(n, per_stream),
                               seed=SEED + i, rng_seed=SEED + i, grid_bits=grid_bits_for(n))
+        if ordered:
+            pos, types = cell_order(spec, pos, types)
         out.append((spec, pos, types))
     return out
 
