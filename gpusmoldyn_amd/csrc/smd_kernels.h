@@ -145,7 +145,7 @@ int sort_passes(uint32_t keyBits, int radixBits);
 size_t sort_counts_words(uint32_t maxN, int radixBits);
 hipError_t launch_sort(SortBuffers& b, const uint32_t* dN, uint32_t maxN, uint32_t keyBits, int radixBits,
                        bool identityVals, hipStream_t stream);
-// first: first_words(nb) words (first[0..nb] and the span starts behind them).
+// first: first_words(nb) words (first[0..nb]).
 size_t first_words(uint32_t nb);
 hipError_t launch_first(const uint32_t* keys, const uint32_t* dN, uint32_t maxN, uint32_t nb, uint32_t* first,
                         uint32_t* dLive, hipStream_t stream);

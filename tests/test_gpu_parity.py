@@ -40,6 +40,7 @@ def test_mt_matches_reference_dcmt(smd, gpu):
 @pytest.mark.parametrize("n,bits", [(0, 8), (1, 8), (4095, 12), (4096, 12), (4097, 20), (100003, 24), (1 << 20, 21),
                                     (3000000, 25)])
 def test_sort_pairs_stable(smd, gpu, n, bits, radix_bits):
+    # up to 12.6M keys the sort runs 2048-key tiles, beyond that 4096-key tiles
     banner(f"sort n={n} bits={bits} radix={radix_bits}")
     rng = np.random.default_rng(n + bits)
     keys = rng.integers(0, 1 << bits, size=n, dtype=np.uint64).astype(np.uint32)
@@ -47,6 +48,20 @@ def test_sort_pairs_stable(smd, gpu, n, bits, radix_bits):
         keys[: n // 3] = keys[0]  # long runs of equal keys
     vals = np.arange(n, dtype=np.uint32)
     k, v = smd.sort_pairs(keys, vals, bits, radix_bits, device=gpu)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(k, keys[order])
+    assert np.array_equal(v, vals[order])
+
+
+def test_sort_pairs_large_tiles(smd, gpu):
+    # above 12.6M keys: 4096-key tiles, ragged last tile
+    n, bits = 13_000_001, 23
+    banner(f"sort n={n} bits={bits} radix=8")
+    rng = np.random.default_rng(7)
+    keys = rng.integers(0, 1 << bits, size=n, dtype=np.uint64).astype(np.uint32)
+    keys[1000:200000] = 12345
+    vals = np.arange(n, dtype=np.uint32)
+    k, v = smd.sort_pairs(keys, vals, bits, 8, device=gpu)
     order = np.argsort(keys, kind="stable")
     assert np.array_equal(k, keys[order])
     assert np.array_equal(v, vals[order])
