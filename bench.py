@@ -96,7 +96,7 @@ def timed(e, steps):
 
 
 # stages that are one kernel each, and that kernel's name in the profiles
-SINGLE_KERNEL = {"producer": "k_producer", "candidates": "k_candidates", "resolve": "k_resolve",
+SINGLE_KERNEL = {"producer": "k_producer", "first": "k_first", "candidates": "k_candidates", "resolve": "k_resolve",
                  "scatter": "k_scatter"}
 PROFILE = {"c2": "step_kernels_c2_1M.md", "c3": "step_kernels_c3_4M.md", "c4": "step_kernels_c4_9M.md"}
 

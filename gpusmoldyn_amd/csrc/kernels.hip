@@ -282,6 +282,7 @@ template <int BITS, int ITEMS>
 __global__ __launch_bounds__(256) void k_sort_rowscan(uint32_t* __restrict__ counts, uint32_t* __restrict__ totals,
                                                       const uint32_t* __restrict__ dN, uint32_t stride) {
   constexpr uint32_t TILE = 256u * ITEMS;
+  constexpr int PER = 8;  // consecutive tile counters per thread
   __shared__ uint32_t wsum[4];
   __shared__ uint32_t tot;
   const uint32_t n = min(*dN, (stride - 1u) * TILE);
@@ -289,11 +290,20 @@ __global__ __launch_bounds__(256) void k_sort_rowscan(uint32_t* __restrict__ cou
   const uint32_t d = blockIdx.x;
   uint32_t* row = counts + (size_t)d * stride;
   uint32_t carry = 0;
-  for (uint32_t b = 0; b < nTiles; b += 256) {
-    const uint32_t e = b + threadIdx.x;
-    const uint32_t v = (e < nTiles) ? row[e] : 0u;
-    const uint32_t ex = block_exscan256(v, wsum, &tot);
-    if (e < nTiles) row[e] = carry + ex;
+  for (uint32_t b = 0; b < nTiles; b += 256 * PER) {
+    const uint32_t e0 = b + threadIdx.x * PER;
+    uint32_t v[PER], sum = 0;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      v[i] = (e0 + i < nTiles) ? row[e0 + i] : 0u;
+      sum += v[i];
+    }
+    uint32_t run = carry + block_exscan256(sum, wsum, &tot);
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      if (e0 + i < nTiles) row[e0 + i] = run;
+      run += v[i];
+    }
     carry += tot;
     __syncthreads();
   }
@@ -408,14 +418,37 @@ __global__ __launch_bounds__(256) void k_sort_down(const uint32_t* __restrict__ 
   }
   __syncthreads();
   const uint32_t tileN = min(TILE, n - tile * TILE);
+  if (GATHER) {
+    // every record load of the thread in flight before the first store
+    uint32_t dk[ITEMS], dv[ITEMS], dd[ITEMS];
+    float4 rec[ITEMS];
+#pragma unroll
+    for (int q = 0; q < ITEMS; ++q) {
+      const uint32_t idx = t + q * 256u;
+      const bool ok = idx < tileN;
+      const uint32_t k = ok ? s_key[idx] : 0xFFFFFFFFu;
+      const uint32_t d = (k >> shift) & (RADIX - 1);
+      dk[q] = k;
+      dv[q] = ok ? s_val[idx] : 0u;
+      dd[q] = ok ? s_base[d] + (idx - s_tstart[d]) : 0u;
+      rec[q] = (ok && k < gatherDeadKey) ? gatherSrc[dv[q]] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int q = 0; q < ITEMS; ++q) {
+      if (t + q * 256u < tileN) {
+        keysOut[dd[q]] = dk[q];
+        valsOut[dd[q]] = dv[q];
+        if (dk[q] < gatherDeadKey) gatherDst[dd[q]] = rec[q];
+      }
+    }
+    return;
+  }
   for (uint32_t idx = t; idx < tileN; idx += 256) {
     const uint32_t k = s_key[idx];
     const uint32_t d = (k >> shift) & (RADIX - 1);
     const uint32_t dst = s_base[d] + (idx - s_tstart[d]);
-    const uint32_t v = s_val[idx];
     keysOut[dst] = k;
-    valsOut[dst] = v;
-    if (GATHER && k < gatherDeadKey) gatherDst[dst] = gatherSrc[v];
+    valsOut[dst] = s_val[idx];
   }
 }
 
@@ -786,11 +819,12 @@ __device__ __forceinline__ void wave_search(const ReplicaTables& T, const CandAr
   const uint32_t key0 = __shfl(key, 0, 64), key63 = __shfl(key, 63, 64);
   if (!mask) return;
   const uint32_t* __restrict__ first = a.first;
-  // own bucket, outside the wave
-  if (key == key0 && wbase > 0 && a.keys[wbase - 1] == key)
-    scan_range(T, a.sortedPT, first[key], wbase, j, p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
-  if (key == key63 && wbase + 64 < nLive && a.keys[wbase + 64] == key)
-    scan_range(T, a.sortedPT, wbase + 64, first[key + 1], j, p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
+  // the part of the own bucket before the wave (lanes holding the wave's
+  // first key) or after it (lanes holding its last key): [first[key], wbase)
+  // and [wbase + 64, first[key + 1]), empty when the bucket starts or ends
+  // inside the wave
+  const bool spillLo = key == key0 && wbase > 0;
+  const bool spillHi = key == key63 && wbase + 64 < nLive;
   // neighbouring buckets across the faces this particle is near
   const float pc[3] = {p.x, p.y, p.z};
   int lo[3], hi[3];
@@ -802,11 +836,39 @@ __device__ __forceinline__ void wave_search(const ReplicaTables& T, const CandAr
     lo[d] = f < rb ? 1 : 0;
     hi[d] = f > 1.0f - rb ? 1 : 0;
   }
-  if (!(lo[0] | hi[0] | lo[1] | hi[1] | lo[2] | hi[2])) return;
   const uint32_t local = key - T.bucketOff;
   const uint32_t bx = T.gbits[0], by = T.gbits[1];
   const uint32_t mx = (1u << bx) - 1u, my = (1u << by) - 1u, mz = (1u << T.gbits[2]) - 1u;
   const uint32_t gx = local & mx, gy = (local >> bx) & my, gz = local >> (bx + by);
+  if (!(lo[0] & hi[0]) && !(lo[1] & hi[1]) && !(lo[2] & hi[2])) {
+    // at most one neighbour per axis (binding reach below half a cell): the
+    // bucket bounds of the own-bucket spill and of up to 7 neighbours are
+    // loaded together, then the ranges are scanned
+    const uint32_t act = (uint32_t)(lo[0] | hi[0]) | ((uint32_t)(lo[1] | hi[1]) << 1) |
+                         ((uint32_t)(lo[2] | hi[2]) << 2);
+    const uint32_t xs = (gx + (hi[0] ? 1u : mx)) & mx;
+    const uint32_t ys = (gy + (hi[1] ? 1u : my)) & my;
+    const uint32_t zs = (gz + (hi[2] ? 1u : mz)) & mz;
+    const uint32_t oLo = spillLo ? first[key] : wbase;
+    const uint32_t oHi = spillHi ? first[key + 1] : wbase + 64;
+    uint32_t fa[7], fb[7];
+#pragma unroll
+    for (int c = 1; c < 8; ++c) {
+      const bool use = ((uint32_t)c & ~act) == 0u;
+      const uint32_t xx = (c & 1) ? xs : gx, yy = (c & 2) ? ys : gy, zz = (c & 4) ? zs : gz;
+      const uint32_t bk = T.bucketOff + (zz << (bx + by)) + (yy << bx) + xx;
+      fa[c - 1] = use ? first[bk] : 0u;
+      fb[c - 1] = use ? first[bk + 1u] : 0u;
+    }
+    if (spillLo) scan_range(T, a.sortedPT, oLo, wbase, j, p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
+    if (spillHi) scan_range(T, a.sortedPT, wbase + 64, oHi, j, p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
+#pragma unroll
+    for (int c = 0; c < 7; ++c)
+      scan_range(T, a.sortedPT, fa[c], fb[c], j, p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
+    return;
+  }
+  if (spillLo) scan_range(T, a.sortedPT, first[key], wbase, j, p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
+  if (spillHi) scan_range(T, a.sortedPT, wbase + 64, first[key + 1], j, p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
   for (int dz = -lo[2]; dz <= hi[2]; ++dz) {
     const uint32_t zz = (gz + (uint32_t)dz) & mz;
     for (int dy = -lo[1]; dy <= hi[1]; ++dy) {
