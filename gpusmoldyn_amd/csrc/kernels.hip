@@ -931,57 +931,70 @@ __global__ __launch_bounds__(kBS) void k_resolve(ResolveArgs a) {
   Mt m;
   bool loaded = false;
   uint32_t deaths = 0;
-  for (uint32_t j = lo + s; j < hi; j += S) {
-    const int c = a.cand[j];
-    if (c <= (int)j) continue;
-    if (a.cand[c] != (int)j) continue;
-    const float4 p1 = a.sortedPT[j];
-    const float4 p2 = a.sortedPT[c];
-    const int t1 = __float_as_int(p1.w), t2 = __float_as_int(p2.w);
-    const int rx = T.pairRxn[t1 * kMaxTypes + t2];
-    if (!loaded) {
-      mt_load<kBS>(m, &lds_mt[threadIdx.x], a.mtState, a.mtPar, a.Stot, g);
-      loaded = true;
-    }
-    const float pr = T.prob[rx];
-    if (!(pr == 1.0f || rand_real(mt_draw<kBS>(m)) < pr)) continue;
-    const float dc1 = T.D[t1], dc2 = T.D[t2];
-    const float x = (dc1 == 0.0f && dc2 == 0.0f) ? 0.5f : dc2 / (dc1 + dc2);
-    // pos2 taken at its image nearest pos1 on periodic axes
-    const float q0 = nearest_image(p1.x, p2.x, T, 0);
-    const float q1 = nearest_image(p1.y, p2.y, T, 1);
-    const float q2 = nearest_image(p1.z, p2.z, T, 2);
-    const float omx = 1.0f - x;
-    float rp[3] = {fmaf(x, p1.x, omx * q0), fmaf(x, p1.y, omx * q1), fmaf(x, p1.z, omx * q2)};
-    const float th = smd_theta(rand_real(mt_draw<kBS>(m)));
-    const float ph = rand_real(mt_draw<kBS>(m)) * 6.28318548f;
-    const float ch = rand_real(mt_draw<kBS>(m)) * 6.28318548f;
-    float v1[3], v2[3];
-    rotate_offsets(th, ph, ch, T.prodPos[rx], v1, v2);
-    const uint32_t mol1 = a.vals[j], mol2 = a.vals[c];
-    float n1[3] = {rp[0] + v1[0], rp[1] + v1[1], rp[2] + v1[2]};
-    float n2[3] = {rp[0] + v2[0], rp[1] + v2[1], rp[2] + v2[2]};
+  // candidates of four consecutive particles of the stream are loaded
+  // together; the particles are still handled in increasing j
+  constexpr int kAhead = 4;
+  int cq[kAhead];
+  for (uint32_t j0 = lo + s; j0 < hi; j0 += kAhead * S) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      if (T.periodic[d]) {
-        if (n1[d] >= T.hi[d]) n1[d] -= T.L[d];
-        else if (n1[d] < T.lo[d]) n1[d] += T.L[d];
-        if (n2[d] >= T.hi[d]) n2[d] -= T.L[d];
-        else if (n2[d] < T.lo[d]) n2[d] += T.L[d];
+    for (int q = 0; q < kAhead; ++q) {
+      const uint32_t jq = j0 + (uint32_t)q * S;
+      cq[q] = jq < hi ? a.cand[jq] : -1;
+    }
+#pragma unroll
+    for (int q = 0; q < kAhead; ++q) {
+      const uint32_t j = j0 + (uint32_t)q * S;
+      const int c = cq[q];
+      if (c <= (int)j) continue;
+      if (a.cand[c] != (int)j) continue;
+      const float4 p1 = a.sortedPT[j];
+      const float4 p2 = a.sortedPT[c];
+      const int t1 = __float_as_int(p1.w), t2 = __float_as_int(p2.w);
+      const int rx = T.pairRxn[t1 * kMaxTypes + t2];
+      if (!loaded) {
+        mt_load<kBS>(m, &lds_mt[threadIdx.x], a.mtState, a.mtPar, a.Stot, g);
+        loaded = true;
       }
-    }
-    const int nt1 = T.prodX[rx], nt2 = T.prodY[rx];
-    a.pos[mol1] = make_float4(n1[0], n1[1], n1[2], 0.0f);
-    a.pos[mol2] = make_float4(n2[0], n2[1], n2[2], 0.0f);
-    a.type[mol1] = nt1;
-    a.type[mol2] = nt2;
-    if (nt1 <= 0) {
-      ++deaths;
-      atomicSub(&a.tileAlive[mol1 / kCompactTile], 1u);
-    }
-    if (nt2 <= 0) {
-      ++deaths;
-      atomicSub(&a.tileAlive[mol2 / kCompactTile], 1u);
+      const float pr = T.prob[rx];
+      if (!(pr == 1.0f || rand_real(mt_draw<kBS>(m)) < pr)) continue;
+      const float dc1 = T.D[t1], dc2 = T.D[t2];
+      const float x = (dc1 == 0.0f && dc2 == 0.0f) ? 0.5f : dc2 / (dc1 + dc2);
+      // pos2 taken at its image nearest pos1 on periodic axes
+      const float q0 = nearest_image(p1.x, p2.x, T, 0);
+      const float q1 = nearest_image(p1.y, p2.y, T, 1);
+      const float q2 = nearest_image(p1.z, p2.z, T, 2);
+      const float omx = 1.0f - x;
+      float rp[3] = {fmaf(x, p1.x, omx * q0), fmaf(x, p1.y, omx * q1), fmaf(x, p1.z, omx * q2)};
+      const float th = smd_theta(rand_real(mt_draw<kBS>(m)));
+      const float ph = rand_real(mt_draw<kBS>(m)) * 6.28318548f;
+      const float ch = rand_real(mt_draw<kBS>(m)) * 6.28318548f;
+      float v1[3], v2[3];
+      rotate_offsets(th, ph, ch, T.prodPos[rx], v1, v2);
+      const uint32_t mol1 = a.vals[j], mol2 = a.vals[c];
+      float n1[3] = {rp[0] + v1[0], rp[1] + v1[1], rp[2] + v1[2]};
+      float n2[3] = {rp[0] + v2[0], rp[1] + v2[1], rp[2] + v2[2]};
+  #pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        if (T.periodic[d]) {
+          if (n1[d] >= T.hi[d]) n1[d] -= T.L[d];
+          else if (n1[d] < T.lo[d]) n1[d] += T.L[d];
+          if (n2[d] >= T.hi[d]) n2[d] -= T.L[d];
+          else if (n2[d] < T.lo[d]) n2[d] += T.L[d];
+        }
+      }
+      const int nt1 = T.prodX[rx], nt2 = T.prodY[rx];
+      a.pos[mol1] = make_float4(n1[0], n1[1], n1[2], 0.0f);
+      a.pos[mol2] = make_float4(n2[0], n2[1], n2[2], 0.0f);
+      a.type[mol1] = nt1;
+      a.type[mol2] = nt2;
+      if (nt1 <= 0) {
+        ++deaths;
+        atomicSub(&a.tileAlive[mol1 / kCompactTile], 1u);
+      }
+      if (nt2 <= 0) {
+        ++deaths;
+        atomicSub(&a.tileAlive[mol2 / kCompactTile], 1u);
+      }
     }
   }
   if (loaded) mt_store<kBS>(m, a.mtState, a.Stot, g);
@@ -1013,6 +1026,7 @@ __device__ __forceinline__ uint32_t block_exscan1024(uint32_t v, uint32_t* wsum,
 __global__ __launch_bounds__(1024) void k_finalize(FinalizeArgs a) {
   __shared__ uint32_t wsum[16];
   __shared__ uint32_t tot;
+  const uint32_t n = *a.dN;  // read before any store (the arguments may alias)
   // replicas
   uint32_t cN = 0, cA = 0, cB = 0;
   for (uint32_t b = 0; b < (uint32_t)a.R; b += 1024) {
@@ -1053,7 +1067,6 @@ __global__ __launch_bounds__(1024) void k_finalize(FinalizeArgs a) {
     cB += tB;
     __syncthreads();
   }
-  const uint32_t n = *a.dN;
   if (threadIdx.x == 0) {
     a.newOff[a.R] = cN;
     *a.dNnext = cN;
@@ -1066,19 +1079,22 @@ __global__ __launch_bounds__(1024) void k_finalize(FinalizeArgs a) {
   for (uint32_t b = 0; b < nTiles; b += 1024 * kTPT) {
     const uint32_t e0 = b + threadIdx.x * kTPT;
     uint32_t va[kTPT], vb[kTPT], sa = 0, sbv = 0;
+    // every load before the first store (the counters are reset here)
 #pragma unroll
     for (uint32_t k = 0; k < kTPT; ++k) {
       const uint32_t e = e0 + k;
-      va[k] = 0;
-      vb[k] = 0;
+      va[k] = e < nTiles ? a.tileAlive[e] : 0u;
+      vb[k] = e < nTiles ? a.tileBirth[e] : 0u;
+      sa += va[k];
+      sbv += vb[k];
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kTPT; ++k) {
+      const uint32_t e = e0 + k;
       if (e < nTiles) {
-        va[k] = a.tileAlive[e];
-        vb[k] = a.tileBirth[e];
         a.tileAlive[e] = 0;
         a.tileBirth[e] = 0;
       }
-      sa += va[k];
-      sbv += vb[k];
     }
     const uint32_t exA = block_exscan1024(sa, wsum, &tot);
     const uint32_t tA = tot;
