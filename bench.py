@@ -34,8 +34,8 @@ def stage_bytes(name, n, S, nb, passes):
     """Algorithmic bytes of one launch of each stage for n particles, S RNG
     streams, nb buckets (compulsory reads + writes, each datum once)."""
     if name == "producer":
-        # pos 16 + type 4 in; pos 16, type 4, key 4, birth type 4, packed record 16 out; stream state
-        return 64 * n + 172 * S
+        # record (x, y, z, type) 16 in; record 16, key 4, birth type 4 out; stream state
+        return 40 * n + 172 * S
     if name == "sort":
         # per digit pass: count (key 4) + scatter (key, value 8 in, 8 out); first pass values implicit;
         # the last pass also copies the packed record of each particle into sorted order (16 + 16)

@@ -39,6 +39,47 @@ static void dfree(T*& p) {
 
 static inline uint32_t round64(uint32_t x) { return (x + 63u) & ~63u; }
 
+// Caller layout (pos4 = x, y, z, w; int types) to and from the canonical
+// device layout (pt = x, y, z, type bits; w apart).
+static void pack_particles(const float* pos4, const int* types, size_t n, std::vector<float4>& pt,
+                           std::vector<float>& w) {
+  pt.resize(n);
+  w.resize(n);
+  for (size_t i = 0; i < n; ++i) {
+    float tb;
+    std::memcpy(&tb, &types[i], sizeof(float));
+    pt[i] = make_float4(pos4[4 * i], pos4[4 * i + 1], pos4[4 * i + 2], tb);
+    w[i] = pos4[4 * i + 3];
+  }
+}
+
+static hipError_t upload_particles(const float* pos4, const int* types, size_t n, float4* dpt, float* dw) {
+  std::vector<float4> pt;
+  std::vector<float> w;
+  pack_particles(pos4, types, n, pt, w);
+  hipError_t e = hipMemcpy(dpt, pt.data(), n * sizeof(float4), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dw, w.data(), n * sizeof(float), hipMemcpyHostToDevice);
+  return e;
+}
+
+static hipError_t download_particles(const float4* dpt, const float* dw, size_t n, float* pos4, int* types) {
+  std::vector<float4> pt(n);
+  std::vector<float> w(pos4 ? n : 0);
+  hipError_t e = hipMemcpy(pt.data(), dpt, n * sizeof(float4), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && pos4) e = hipMemcpy(w.data(), dw, n * sizeof(float), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return e;
+  for (size_t i = 0; i < n; ++i) {
+    if (pos4) {
+      pos4[4 * i] = pt[i].x;
+      pos4[4 * i + 1] = pt[i].y;
+      pos4[4 * i + 2] = pt[i].z;
+      pos4[4 * i + 3] = w[i];
+    }
+    if (types) std::memcpy(&types[i], &pt[i].w, sizeof(int));
+  }
+  return hipSuccess;
+}
+
 Engine::Engine(int dev, int rbits) : device(dev), radixBits(rbits == 4 || rbits == 8 || rbits == 11 ? rbits : 0) {}
 
 Engine::~Engine() {
@@ -87,14 +128,16 @@ Engine::~Engine() {
 
 void Engine::free_particle_buffers() {
   for (int k = 0; k < 2; ++k) {
-    dfree(dPos[k]);
-    dfree(dType[k]);
+    dfree(dPT[k]);
+    dfree(dW[k]);
   }
   dfree(dAddedPos);
   dfree(dBirthType);
   dfree(dSortedPT);
-  dfree(dPType);
   dfree(dCand);
+  dfree(dViewPos);
+  dfree(dViewType);
+  viewCap = 0;
   dfree(sb.keys);
   dfree(sb.vals);
   dfree(sb.keysAlt);
@@ -134,13 +177,12 @@ int Engine::add_replica(const smd_sim_spec& spec, const float* pos4, const int* 
 smd_status Engine::alloc_particle_buffers(uint32_t c) {
   cap = c;
   for (int k = 0; k < 2; ++k) {
-    HIPCHK(dalloc(&dPos[k], cap));
-    HIPCHK(dalloc(&dType[k], cap));
+    HIPCHK(dalloc(&dPT[k], cap));
+    HIPCHK(dalloc(&dW[k], cap));
   }
   HIPCHK(dalloc(&dAddedPos, cap));
   HIPCHK(dalloc(&dBirthType, cap));
   HIPCHK(dalloc(&dSortedPT, cap));
-  HIPCHK(dalloc(&dPType, cap));
   HIPCHK(dalloc(&dCand, cap));
   HIPCHK(dalloc(&sb.keys, cap));
   HIPCHK(dalloc(&sb.vals, cap));
@@ -320,8 +362,7 @@ smd_status Engine::finalize(uint32_t capacity) {
   if (st != SMD_OK) return st;
   for (uint32_t r = 0; r < R; ++r) {
     if (cnt[r] == 0) continue;
-    HIPCHK(hipMemcpy(dPos[0] + off[r], pos0[r].data(), cnt[r] * sizeof(float4), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(dType[0] + off[r], type0[r].data(), cnt[r] * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(upload_particles(pos0[r].data(), type0[r].data(), cnt[r], dPT[0] + off[r], dW[0] + off[r]));
   }
   cur = 0;
   Nub = total;
@@ -363,24 +404,24 @@ smd_status Engine::grow(uint64_t need) {
     return SMD_ERR_CAPACITY;
   }
   const uint32_t n = (uint32_t)knownN;
-  float4* keepPos = nullptr;
-  int* keepType = nullptr;
-  HIPCHK(dalloc(&keepPos, n));
-  HIPCHK(dalloc(&keepType, n));
+  float4* keepPT = nullptr;
+  float* keepW = nullptr;
+  HIPCHK(dalloc(&keepPT, n));
+  HIPCHK(dalloc(&keepW, n));
   if (n) {
-    HIPCHK(hipMemcpy(keepPos, dPos[cur], n * sizeof(float4), hipMemcpyDeviceToDevice));
-    HIPCHK(hipMemcpy(keepType, dType[cur], n * sizeof(int), hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(keepPT, dPT[cur], n * sizeof(float4), hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(keepW, dW[cur], n * sizeof(float), hipMemcpyDeviceToDevice));
   }
   free_particle_buffers();
   smd_status st = alloc_particle_buffers((uint32_t)nc);
   if (st != SMD_OK) return st;
   cur = 0;
   if (n) {
-    HIPCHK(hipMemcpy(dPos[0], keepPos, n * sizeof(float4), hipMemcpyDeviceToDevice));
-    HIPCHK(hipMemcpy(dType[0], keepType, n * sizeof(int), hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(dPT[0], keepPT, n * sizeof(float4), hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(dW[0], keepW, n * sizeof(float), hipMemcpyDeviceToDevice));
   }
-  dfree(keepPos);
-  dfree(keepType);
+  dfree(keepPT);
+  dfree(keepW);
   HIPCHK(hipStreamSynchronize(stream));
   return SMD_OK;
 }
@@ -476,8 +517,8 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
   pa.mtPar = dMtPar;
   pa.Stot = Stot;
   pa.gauss = dGauss;
-  pa.pos = dPos[cur];
-  pa.type = dType[cur];
+  pa.pt = dPT[cur];
+  pa.w = dW[cur];
   pa.addedPos = dAddedPos;
   pa.birthType = dBirthType;
   pa.key = sb.keys;
@@ -487,7 +528,6 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
   pa.tileAlive = dTileAlive;
   pa.tileBirth = dTileBirth;
   pa.retire = dRetire;
-  pa.ptype = dPType;
   HIPCHK(launch_producer(pa, stream));
   if (anyZero) {
     ZeroArgs za;
@@ -503,7 +543,7 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
   }
   mark(1);
   if (anyBim) {
-    sb.gatherSrc = dPType;
+    sb.gatherSrc = dPT[cur];
     sb.gatherDst = dSortedPT;
     sb.gatherDeadKey = deadKey;
     HIPCHK(launch_sort(sb, dN, maxN, keyBits, radixBits, true, stream));
@@ -531,8 +571,8 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
     ra.cand = dCand;
     ra.sortedPT = dSortedPT;
     ra.vals = sb.vals;
-    ra.pos = dPos[cur];
-    ra.type = dType[cur];
+    ra.pt = dPT[cur];
+    ra.w = dW[cur];
     ra.repDeaths = dRepDeaths;
     ra.tileAlive = dTileAlive;
     HIPCHK(launch_resolve(ra, stream));
@@ -574,16 +614,16 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
   sa.repAlive = dRepAlive;
   sa.tileAlivePre = dTileAlivePre;
   sa.tileBirthPre = dTileBirthPre;
-  sa.pos = dPos[cur];
-  sa.type = dType[cur];
+  sa.pt = dPT[cur];
+  sa.w = dW[cur];
   sa.addedPos = dAddedPos;
   sa.birthType = dBirthType;
-  sa.posOut = dPos[cur ^ 1];
-  sa.typeOut = dType[cur ^ 1];
+  sa.ptOut = dPT[cur ^ 1];
+  sa.wOut = dW[cur ^ 1];
   HIPCHK(launch_scatter(sa, maxN, stream));
   if (anyZero)
-    HIPCHK(launch_zero_place((int)R, dTables, dZPos, dZType, dZCount, dNewOff, dZeroBase, dPos[cur ^ 1],
-                             dType[cur ^ 1], stream));
+    HIPCHK(launch_zero_place((int)R, dTables, dZPos, dZType, dZCount, dNewOff, dZeroBase, dPT[cur ^ 1],
+                             dW[cur ^ 1], stream));
   if (injN) {
     InjectArgs ia;
     ia.injPos = dInjPos;
@@ -593,8 +633,8 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
     ia.nInj = injN;
     ia.newOff = dNewOff;
     ia.injBase = dInjBase;
-    ia.posOut = dPos[cur ^ 1];
-    ia.typeOut = dType[cur ^ 1];
+    ia.ptOut = dPT[cur ^ 1];
+    ia.wOut = dW[cur ^ 1];
     HIPCHK(launch_inject(ia, stream));
     injN = 0;
   }
@@ -834,10 +874,7 @@ smd_status Engine::set_particles(const float* pos4, const int* types, uint32_t n
     st = grow(4ull * n);
     if (st != SMD_OK) return st;
   }
-  if (n) {
-    HIPCHK(hipMemcpy(dPos[cur], pos4, n * sizeof(float4), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(dType[cur], types, n * sizeof(int), hipMemcpyHostToDevice));
-  }
+  if (n) HIPCHK(upload_particles(pos4, types, n, dPT[cur], dW[cur]));
   const uint32_t off[2] = {0u, n};
   HIPCHK(hipMemcpy(dRepOff, off, 2 * sizeof(uint32_t), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(dRepCount, &n, sizeof(uint32_t), hipMemcpyHostToDevice));
@@ -873,8 +910,31 @@ smd_status Engine::get(int r, float* pos4, int* types, uint32_t capN, uint32_t* 
   HIPCHK(hipMemcpy(&n, dRepCount + r, sizeof(uint32_t), hipMemcpyDeviceToHost));
   if (nOut) *nOut = n;
   const uint32_t m = std::min(n, capN);
-  if (m && pos4) HIPCHK(hipMemcpy(pos4, dPos[cur] + off, m * sizeof(float4), hipMemcpyDeviceToHost));
-  if (m && types) HIPCHK(hipMemcpy(types, dType[cur] + off, m * sizeof(int), hipMemcpyDeviceToHost));
+  if (m && (pos4 || types)) HIPCHK(download_particles(dPT[cur] + off, dW[cur] + off, m, pos4, types));
+  return SMD_OK;
+}
+
+smd_status Engine::view(float4** pos4, int** types, uint32_t* nOut) {
+  if (!finalized) {
+    set_error("engine not finalized");
+    return SMD_ERR_INVALID;
+  }
+  smd_status st = sync();
+  if (st != SMD_OK) return st;
+  uint32_t n = 0;
+  HIPCHK(hipMemcpy(&n, dN, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (n > viewCap || !dViewPos) {
+    dfree(dViewPos);
+    dfree(dViewType);
+    viewCap = std::max<uint32_t>(n, 1u);
+    HIPCHK(dalloc(&dViewPos, viewCap));
+    HIPCHK(dalloc(&dViewType, viewCap));
+  }
+  HIPCHK(launch_unpack(dPT[cur], dW[cur], n, dViewPos, dViewType, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  *pos4 = dViewPos;
+  *types = dViewType;
+  *nOut = n;
   return SMD_OK;
 }
 
@@ -923,8 +983,8 @@ smd_status Engine::debug_stage(int stage, float* pos4, int* types, uint32_t* key
   pa.mtPar = dMtPar;
   pa.Stot = Stot;
   pa.gauss = dGauss;
-  pa.pos = dPos[cur];
-  pa.type = dType[cur];
+  pa.pt = dPT[cur];
+  pa.w = dW[cur];
   pa.addedPos = dAddedPos;
   pa.birthType = dBirthType;
   pa.key = sb.keys;
@@ -934,11 +994,10 @@ smd_status Engine::debug_stage(int stage, float* pos4, int* types, uint32_t* key
   pa.tileAlive = dTileAlive;
   pa.tileBirth = dTileBirth;
   pa.retire = nullptr;
-  pa.ptype = dPType;
   HIPCHK(launch_producer(pa, stream));
   uint32_t live = 0;
   if (stage >= 2) {
-    sb.gatherSrc = dPType;
+    sb.gatherSrc = dPT[cur];
     sb.gatherDst = dSortedPT;
     sb.gatherDeadKey = deadKey;
     HIPCHK(launch_sort(sb, dN, maxN, keyBits, radixBits, true, stream));
@@ -967,8 +1026,8 @@ smd_status Engine::debug_stage(int stage, float* pos4, int* types, uint32_t* key
     ra.cand = dCand;
     ra.sortedPT = dSortedPT;
     ra.vals = sb.vals;
-    ra.pos = dPos[cur];
-    ra.type = dType[cur];
+    ra.pt = dPT[cur];
+    ra.w = dW[cur];
     ra.repDeaths = dRepDeaths;
     ra.tileAlive = dTileAlive;
     HIPCHK(launch_resolve(ra, stream));
@@ -978,8 +1037,7 @@ smd_status Engine::debug_stage(int stage, float* pos4, int* types, uint32_t* key
   if (nTotal) *nTotal = n;
   if (nLive) *nLive = live;
   if (n) {
-    if (pos4) HIPCHK(hipMemcpy(pos4, dPos[cur], n * sizeof(float4), hipMemcpyDeviceToHost));
-    if (types) HIPCHK(hipMemcpy(types, dType[cur], n * sizeof(int), hipMemcpyDeviceToHost));
+    if (pos4 || types) HIPCHK(download_particles(dPT[cur], dW[cur], n, pos4, types));
     if (stage == 1 && keysSorted) HIPCHK(hipMemcpy(keysSorted, sb.keys, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
     if (stage >= 2 && keysSorted)
       HIPCHK(hipMemcpy(keysSorted, sb.keys, n * sizeof(uint32_t), hipMemcpyDeviceToHost));

@@ -99,31 +99,22 @@ __global__ __launch_bounds__(kBS) void k_producer(ProducerArgs a) {
     // The next particle's position and type are requested before the
     // current one is processed (two loads in flight per lane).
     float4 pNext = make_float4(0.f, 0.f, 0.f, 0.f);
-    int tNext = 0;
-    if (laneOK && s < n) {
-      pNext = a.pos[off + s];
-      tNext = a.type[off + s];
-    }
+    if (laneOK && s < n) pNext = a.pt[off + s];
     for (uint32_t it = 0; it < iters; ++it) {
       const uint32_t il = s + it * S;
       const bool act = laneOK && il < n;
       const uint32_t i = off + il;
       const float4 pCur = pNext;
-      const int tCur = tNext;
-      if (laneOK && il + S < n) {
-        pNext = a.pos[i + S];
-        tNext = a.type[i + S];
-      }
+      if (laneOK && il + S < n) pNext = a.pt[i + S];
       bool alive = false, birth = false;
       if (act && kill) {
-        a.type[i] = 0;
+        a.pt[i] = make_float4(pCur.x, pCur.y, pCur.z, __int_as_float(0));
         a.key[i] = a.deadKey;
         a.birthType[i] = 0;
-        a.ptype[i] = make_float4(pCur.x, pCur.y, pCur.z, __int_as_float(0));
         ++nDeaths;
       } else if (act) {
-        float4 p = pCur;
-        int t = tCur;
+        const float4 p = pCur;
+        int t = __float_as_int(pCur.w);
         uint32_t u0, u1, u2;
         mt_draw3<kBS>(m, u0, u1, u2);
         const float rt = T.rate[t > 0 ? t : 0];
@@ -131,10 +122,10 @@ __global__ __launch_bounds__(kBS) void k_producer(ProducerArgs a) {
         np[0] = fmaf(rt, lds_gauss[u0 & (kGaussDim - 1)], p.x);
         np[1] = fmaf(rt, lds_gauss[u1 & (kGaussDim - 1)], p.y);
         np[2] = fmaf(rt, lds_gauss[u2 & (kGaussDim - 1)], p.z);
-        float w = p.w;
+        bool wZero = false;  // the fourth component becomes 0 (offset products)
         int bt = 0;
         float bp[3] = {np[0], np[1], np[2]};
-        float bw = p.w;
+        float bw = 0.0f;
         if (t <= 0) {
           t = 0;
         } else if (T.hasUni) {
@@ -162,8 +153,9 @@ __global__ __launch_bounds__(kBS) void k_producer(ProducerArgs a) {
                     np[0] = np[0] + v1[0];
                     np[1] = np[1] + v1[1];
                     np[2] = np[2] + v1[2];
-                    w = 0.0f;
-                    bw = 0.0f;
+                    wZero = true;
+                  } else {
+                    bw = a.w[i];  // the birth inherits the parent's fourth component
                   }
                 }
               }
@@ -181,9 +173,8 @@ __global__ __launch_bounds__(kBS) void k_producer(ProducerArgs a) {
             key = bucketOff + grid_hash(T, np[0], np[1], np[2]);
           }
         }
-        a.pos[i] = make_float4(np[0], np[1], np[2], w);
-        a.type[i] = t;
-        a.ptype[i] = make_float4(np[0], np[1], np[2], __int_as_float(t));
+        a.pt[i] = make_float4(np[0], np[1], np[2], __int_as_float(t));
+        if (wZero) a.w[i] = 0.0f;
         a.key[i] = key;
         a.birthType[i] = bt;
         if (bt) a.addedPos[i] = make_float4(bp[0], bp[1], bp[2], bw);
@@ -983,10 +974,10 @@ __global__ __launch_bounds__(kBS) void k_resolve(ResolveArgs a) {
         }
       }
       const int nt1 = T.prodX[rx], nt2 = T.prodY[rx];
-      a.pos[mol1] = make_float4(n1[0], n1[1], n1[2], 0.0f);
-      a.pos[mol2] = make_float4(n2[0], n2[1], n2[2], 0.0f);
-      a.type[mol1] = nt1;
-      a.type[mol2] = nt2;
+      a.pt[mol1] = make_float4(n1[0], n1[1], n1[2], __int_as_float(nt1));
+      a.pt[mol2] = make_float4(n2[0], n2[1], n2[2], __int_as_float(nt2));
+      a.w[mol1] = 0.0f;
+      a.w[mol2] = 0.0f;
       if (nt1 <= 0) {
         ++deaths;
         atomicSub(&a.tileAlive[mol1 / kCompactTile], 1u);
@@ -1134,12 +1125,14 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
   constexpr uint32_t kWaveSpan = kCompactTile / 4;  // consecutive elements per wave
   bool al[kItems], bi[kItems];
   uint32_t ra[kItems], rb[kItems];
+  float4 pv[kItems];
   uint32_t ca = 0, cb = 0;
 #pragma unroll
   for (int it = 0; it < kItems; ++it) {
     const uint32_t e = tile * kCompactTile + w * kWaveSpan + it * 64 + lane;
     const bool v = e < n;
-    al[it] = v && a.type[e] > 0;
+    pv[it] = v ? a.pt[e] : make_float4(0.f, 0.f, 0.f, 0.f);
+    al[it] = v && __float_as_int(pv[it].w) > 0;
     bi[it] = v && a.birthType[e] > 0;
     const uint64_t am = __ballot(al[it]);
     const uint64_t bm = __ballot(bi[it]);
@@ -1178,13 +1171,14 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
     while (r + 1 < a.R && a.repOff[r + 1] <= e) ++r;
     if (al[it]) {
       const uint32_t dst = a.newOff[r] + (pa + ra[it] - a.aliveBase[r]);
-      a.posOut[dst] = a.pos[e];
-      a.typeOut[dst] = a.type[e];
+      a.ptOut[dst] = pv[it];
+      a.wOut[dst] = a.w[e];
     }
     if (bi[it]) {
       const uint32_t dst = a.newOff[r] + a.repAlive[r] + (pb + rb[it] - a.birthBase[r]);
-      a.posOut[dst] = a.addedPos[e];
-      a.typeOut[dst] = a.birthType[e];
+      const float4 q = a.addedPos[e];
+      a.ptOut[dst] = make_float4(q.x, q.y, q.z, __int_as_float(a.birthType[e]));
+      a.wOut[dst] = q.w;
     }
   }
 }
@@ -1199,8 +1193,9 @@ __global__ __launch_bounds__(256) void k_inject(InjectArgs a) {
   if (q >= a.nInj) return;
   const int r = a.injRep[q];
   const uint32_t dst = a.newOff[r] + a.injBase[r] + (q - a.injSrcOff[r]);
-  a.posOut[dst] = a.injPos[q];
-  a.typeOut[dst] = a.injType[q];
+  const float4 p = a.injPos[q];
+  a.ptOut[dst] = make_float4(p.x, p.y, p.z, __int_as_float(a.injType[q]));
+  a.wOut[dst] = p.w;
 }
 
 // ---------------------------------------------------------------------------
@@ -1261,16 +1256,26 @@ __global__ __launch_bounds__(64) void k_zero_place(const ReplicaTables* __restri
                                                    const float4* __restrict__ zPos, const int* __restrict__ zType,
                                                    const uint32_t* __restrict__ zCount,
                                                    const uint32_t* __restrict__ newOff,
-                                                   const uint32_t* __restrict__ zeroBase, float4* __restrict__ posOut,
-                                                   int* __restrict__ typeOut) {
+                                                   const uint32_t* __restrict__ zeroBase, float4* __restrict__ ptOut,
+                                                   float* __restrict__ wOut) {
   const int r = blockIdx.x;
   const uint32_t c = zCount[r];
   if (c == 0) return;
   const uint32_t src = tables[r].zeroOff, dst = newOff[r] + zeroBase[r];
   for (uint32_t j = threadIdx.x; j < c; j += 64) {
-    posOut[dst + j] = zPos[src + j];
-    typeOut[dst + j] = zType[src + j];
+    const float4 p = zPos[src + j];
+    ptOut[dst + j] = make_float4(p.x, p.y, p.z, __int_as_float(zType[src + j]));
+    wOut[dst + j] = p.w;
   }
+}
+
+__global__ __launch_bounds__(256) void k_unpack(const float4* __restrict__ pt, const float* __restrict__ w, uint32_t n,
+                                                float4* __restrict__ pos4, int* __restrict__ type) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float4 p = pt[i];
+  pos4[i] = make_float4(p.x, p.y, p.z, w[i]);
+  type[i] = __float_as_int(p.w);
 }
 
 __global__ __launch_bounds__(256) void k_mt_init_range(uint32_t* __restrict__ st, const uint32_t* __restrict__ seeds,
@@ -1476,11 +1481,17 @@ hipError_t launch_zero(const ZeroArgs& a, hipStream_t stream) {
 }
 
 hipError_t launch_zero_place(int R, const ReplicaTables* tables, const float4* zPos, const int* zType,
-                             const uint32_t* zCount, const uint32_t* newOff, const uint32_t* zeroBase, float4* posOut,
-                             int* typeOut, hipStream_t stream) {
+                             const uint32_t* zCount, const uint32_t* newOff, const uint32_t* zeroBase, float4* ptOut,
+                             float* wOut, hipStream_t stream) {
   if (R <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_zero_place, dim3(R), dim3(64), 0, stream, tables, zPos, zType, zCount, newOff, zeroBase,
-                     posOut, typeOut);
+                     ptOut, wOut);
+  return hipGetLastError();
+}
+
+hipError_t launch_unpack(const float4* pt, const float* w, uint32_t n, float4* pos4, int* type, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_unpack, dim3(cdiv(n, 256)), dim3(256), 0, stream, pt, w, n, pos4, type);
   return hipGetLastError();
 }
 

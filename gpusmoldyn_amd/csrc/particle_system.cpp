@@ -205,8 +205,19 @@ void ParticleSystem::reset(ParticleConfig config) {
   m_status = m_engine->impl->set_particles(m_hPos.data(), types.data(), n);
 }
 
-void* ParticleSystem::getCudaPosVBO() const { return m_engine ? (void*)m_engine->impl->current_pos() : nullptr; }
-void* ParticleSystem::getCudaTypesVBO() const { return m_engine ? (void*)m_engine->impl->current_type() : nullptr; }
+void* ParticleSystem::getCudaPosVBO() const {
+  float4* p = nullptr;
+  int* t = nullptr;
+  uint32_t n = 0;
+  return (m_engine && m_engine->impl->view(&p, &t, &n) == SMD_OK) ? (void*)p : nullptr;
+}
+
+void* ParticleSystem::getCudaTypesVBO() const {
+  float4* p = nullptr;
+  int* t = nullptr;
+  uint32_t n = 0;
+  return (m_engine && m_engine->impl->view(&p, &t, &n) == SMD_OK) ? (void*)t : nullptr;
+}
 
 void ParticleSystem::dumpGrid() {
   // particleSystem.cpp:1003-1017 (largest cell occupancy), on the host

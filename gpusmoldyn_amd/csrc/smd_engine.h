@@ -31,8 +31,10 @@ class Engine {
   smd_status particle_steps(unsigned long long* out);
   // Replace the particles of a single-replica engine (setArray / reset).
   smd_status set_particles(const float* pos4, const int* types, uint32_t n);
-  float4* current_pos() const { return dPos[cur]; }
-  int* current_type() const { return dType[cur]; }
+  // Device arrays of the current particles in the caller layout (pos4 =
+  // (x, y, z, w), int types), unpacked from the canonical state into view
+  // buffers that stay valid until the next step or view call.
+  smd_status view(float4** pos4, int** types, uint32_t* n);
 
   int device;
   int radixBits;
@@ -63,13 +65,17 @@ class Engine {
   uint32_t *dMtState = nullptr, *dMtPar = nullptr;
   float* dGauss = nullptr;
   uint32_t* dFirst = nullptr;
-  float4* dPos[2] = {nullptr, nullptr};
-  int* dType[2] = {nullptr, nullptr};
+  // canonical particle state, ping-ponged by the compaction: (x, y, z, type
+  // bits) and the fourth position component
+  float4* dPT[2] = {nullptr, nullptr};
+  float* dW[2] = {nullptr, nullptr};
   float4* dAddedPos = nullptr;
   int* dBirthType = nullptr;
   float4* dSortedPT = nullptr;
-  float4* dPType = nullptr;  // (x, y, z, type bits) after the producer, canonical order
   int* dCand = nullptr;
+  float4* dViewPos = nullptr;  // view() buffers
+  int* dViewType = nullptr;
+  uint32_t viewCap = 0;
   SortBuffers sb = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 1, nullptr, nullptr, 0};
   uint32_t *dTileAlive = nullptr, *dTileBirth = nullptr, *dTileAlivePre = nullptr, *dTileBirthPre = nullptr;
   uint32_t* dStatus = nullptr;  // sticky kernel error bits

@@ -16,8 +16,8 @@ struct ProducerArgs {
   const uint32_t* mtPar;      // [3][Stot]
   uint32_t Stot;
   const float* gauss;         // 4096-entry table
-  float4* pos;                // canonical arrays, updated in place
-  int* type;
+  float4* pt;                 // canonical (x, y, z, type bits), updated in place; also the pair-search source
+  float* w;                   // canonical fourth position component (set to 0 for offset products)
   float4* addedPos;
   int* birthType;
   uint32_t* key;
@@ -27,7 +27,6 @@ struct ProducerArgs {
   uint32_t* tileAlive;
   uint32_t* tileBirth;
   const uint32_t* retire;     // per replica: remove every particle this step (nullptr: none)
-  float4* ptype;              // (x, y, z, type bits) after the move, canonical order (pair-search input)
 };
 
 struct SortBuffers {
@@ -66,8 +65,8 @@ struct ResolveArgs {
   const int* cand;
   const float4* sortedPT;
   const uint32_t* vals;
-  float4* pos;
-  int* type;
+  float4* pt;
+  float* w;
   uint32_t* repDeaths;
   uint32_t* tileAlive;
 };
@@ -115,8 +114,8 @@ struct InjectArgs {
   uint32_t nInj;
   const uint32_t* newOff;
   const uint32_t* injBase;
-  float4* posOut;
-  int* typeOut;
+  float4* ptOut;
+  float* wOut;
 };
 
 struct ScatterArgs {
@@ -129,12 +128,12 @@ struct ScatterArgs {
   const uint32_t* repAlive;
   const uint32_t* tileAlivePre;
   const uint32_t* tileBirthPre;
-  const float4* pos;
-  const int* type;
+  const float4* pt;
+  const float* w;
   const float4* addedPos;
   const int* birthType;
-  float4* posOut;
-  int* typeOut;
+  float4* ptOut;
+  float* wOut;
 };
 
 hipError_t launch_mt_init(uint32_t* st, const uint32_t* seeds, uint32_t S, hipStream_t stream);
@@ -159,8 +158,10 @@ hipError_t launch_inject(const InjectArgs& a, hipStream_t stream);
 hipError_t launch_zero(const ZeroArgs& a, hipStream_t stream);
 // Appends replica r's created particles after its survivors and births.
 hipError_t launch_zero_place(int R, const ReplicaTables* tables, const float4* zPos, const int* zType,
-                             const uint32_t* zCount, const uint32_t* newOff, const uint32_t* zeroBase, float4* posOut,
-                             int* typeOut, hipStream_t stream);
+                             const uint32_t* zCount, const uint32_t* newOff, const uint32_t* zeroBase, float4* ptOut,
+                             float* wOut, hipStream_t stream);
+// pos4[i] = (pt[i].xyz, w[i]), type[i] = type bits of pt[i], i < n.
+hipError_t launch_unpack(const float4* pt, const float* w, uint32_t n, float4* pos4, int* type, hipStream_t stream);
 hipError_t launch_mt_init_range(uint32_t* st, const uint32_t* seeds, uint32_t Stot, uint32_t off, uint32_t count,
                                 hipStream_t stream);
 

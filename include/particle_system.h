@@ -45,8 +45,10 @@ class ParticleSystem {
   float getSystemVolume() const { return m_spec.volume; }
   unsigned int getPosBuffer() const { return 0; }
   unsigned int getTypesBuffer() const { return 0; }
-  void* getCudaPosVBO() const;    // device positions of the current step
-  void* getCudaTypesVBO() const;  // device types of the current step
+  // device positions (float4) / types (int) of the current step: copies
+  // that stay valid until the next update() or VBO call
+  void* getCudaPosVBO() const;
+  void* getCudaTypesVBO() const;
 
   void dumpGrid();
   void dumpParticles(unsigned int start, unsigned int count);
