@@ -96,16 +96,24 @@ __global__ __launch_bounds__(kBS) void k_producer(ProducerArgs a) {
     Mt m;
     if (laneOK && !kill) mt_load<kBS>(m, &lds_mt[threadIdx.x], a.mtState, a.mtPar, a.Stot, g);
     const uint32_t iters = (n > s0) ? (n - s0 + S - 1) / S : 0;
-    // The next particle's position and type are requested before the
-    // current one is processed (two loads in flight per lane).
-    float4 pNext = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (laneOK && s < n) pNext = a.pt[off + s];
+    // The records of the next kPre particles of the stream are requested
+    // before the current one is processed (kPre loads in flight per lane).
+    constexpr int kPre = 4;
+    float4 pq[kPre];
+#pragma unroll
+    for (int q = 0; q < kPre; ++q) {
+      const uint32_t iq = s + (uint32_t)q * S;
+      pq[q] = (laneOK && iq < n) ? a.pt[off + iq] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
     for (uint32_t it = 0; it < iters; ++it) {
       const uint32_t il = s + it * S;
       const bool act = laneOK && il < n;
       const uint32_t i = off + il;
-      const float4 pCur = pNext;
-      if (laneOK && il + S < n) pNext = a.pt[i + S];
+      const float4 pCur = pq[0];
+#pragma unroll
+      for (int q = 0; q + 1 < kPre; ++q) pq[q] = pq[q + 1];
+      const uint32_t iNext = il + (uint32_t)kPre * S;
+      pq[kPre - 1] = (laneOK && iNext < n) ? a.pt[off + iNext] : make_float4(0.f, 0.f, 0.f, 0.f);
       bool alive = false, birth = false;
       if (act && kill) {
         a.pt[i] = make_float4(pCur.x, pCur.y, pCur.z, __int_as_float(0));
