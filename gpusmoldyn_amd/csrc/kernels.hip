@@ -69,16 +69,44 @@ __global__ __launch_bounds__(kBS) void k_mt_fill(uint32_t* __restrict__ st, cons
 // the particles i = s, s + S, s + 2S, ... of its replica in increasing order,
 // so every stream's draws happen in a fixed order.
 // ---------------------------------------------------------------------------
+// Per-type and per-reaction entries the producer indexes per particle, staged
+// in LDS for each wave's replica (a few hundred bytes per wave).
+struct ProducerLds {
+  float rate[kMaxTypes];
+  int32_t uniCount[kMaxTypes];
+  int32_t uniStart[kMaxTypes];
+  int32_t uniList[kMaxRxn];
+  float prob[kMaxRxn];
+  int32_t prodX[kMaxRxn];
+  int32_t prodY[kMaxRxn];
+};
+
 __global__ __launch_bounds__(kBS) void k_producer(ProducerArgs a) {
   __shared__ uint32_t lds_mt[kMtNN * kBS];
   __shared__ float lds_gauss[kGaussDim];
+  __shared__ ProducerLds lds_tab[kBS / 64];
   for (int k = threadIdx.x; k < kGaussDim; k += kBS) lds_gauss[k] = a.gauss[k];
-  __syncthreads();
 
   const uint32_t g = blockIdx.x * kBS + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63u;
   const int r = (g < a.Stot) ? a.streamReplica[g] : -1;
   const int rw = __builtin_amdgcn_readfirstlane(r);  // a wave never spans two replicas
+  ProducerLds& L = lds_tab[threadIdx.x >> 6];
+  if (rw >= 0) {
+    const ReplicaTables& T0 = a.tables[rw];
+    if (lane < (uint32_t)kMaxTypes) {
+      L.rate[lane] = T0.rate[lane];
+      L.uniCount[lane] = T0.uniCount[lane];
+      L.uniStart[lane] = T0.uniStart[lane];
+    }
+    if (lane < (uint32_t)kMaxRxn) {
+      L.uniList[lane] = T0.uniList[lane];
+      L.prob[lane] = T0.prob[lane];
+      L.prodX[lane] = T0.prodX[lane];
+      L.prodY[lane] = T0.prodY[lane];
+    }
+  }
+  __syncthreads();
   uint32_t nDeaths = 0, nBirths = 0;
 
   if (rw >= 0) {
@@ -125,7 +153,7 @@ __global__ __launch_bounds__(kBS) void k_producer(ProducerArgs a) {
         int t = __float_as_int(pCur.w);
         uint32_t u0, u1, u2;
         mt_draw3<kBS>(m, u0, u1, u2);
-        const float rt = T.rate[t > 0 ? t : 0];
+        const float rt = L.rate[t > 0 ? t : 0];
         float np[3];
         np[0] = fmaf(rt, lds_gauss[u0 & (kGaussDim - 1)], p.x);
         np[1] = fmaf(rt, lds_gauss[u1 & (kGaussDim - 1)], p.y);
@@ -137,16 +165,16 @@ __global__ __launch_bounds__(kBS) void k_producer(ProducerArgs a) {
         if (t <= 0) {
           t = 0;
         } else if (T.hasUni) {
-          const int cnt = T.uniCount[t];
-          const int base = T.uniStart[t];
+          const int cnt = L.uniCount[t];
+          const int base = L.uniStart[t];
           for (int j = 0; j < cnt; ++j) {
-            const int rx = T.uniList[base + j];
-            if (rand_real(mt_draw<kBS>(m)) < T.prob[rx]) {
-              if (T.prodX[rx] <= 0) {
+            const int rx = L.uniList[base + j];
+            if (rand_real(mt_draw<kBS>(m)) < L.prob[rx]) {
+              if (L.prodX[rx] <= 0) {
                 t = 0;
               } else {
-                t = T.prodX[rx];
-                const int t2 = T.prodY[rx];
+                t = L.prodX[rx];
+                const int t2 = L.prodY[rx];
                 if (t2 > 0) {
                   bt = t2;
                   if (T.unbindOffsets) {
