@@ -816,25 +816,48 @@ __device__ __forceinline__ void scan_range(const ReplicaTables& T, const float4*
 // the cell coordinates, so no pair is ever lost to rounding.
 constexpr float kFaceMargin = 1.0f / 64.0f;
 
-// Search of one wave.  Lanes hold consecutive sorted particles, so the
-// particles of a lane's own bucket that sit in the same wave are found by
-// lane shuffles; the part of the bucket before or after the wave, and the
-// neighbouring buckets across near faces, are read from memory.
+// Search of one wave.  Lanes hold consecutive sorted particles.  Along x the
+// buckets of one grid row have consecutive keys, so the own bucket and the
+// x-neighbours a particle needs form one key range [kLo, kHi]: the particles
+// of that range that sit in the same wave are found by lane shuffles, the
+// parts before or after the wave are read from memory, and each other row
+// (y or z neighbour) is one more range of consecutive buckets.  Rows that
+// wrap along x, or a binding reach above half a cell, take the per-bucket
+// search instead.
 __device__ __forceinline__ void wave_search(const ReplicaTables& T, const CandArgs& a, uint32_t j, uint32_t wbase,
                                             uint32_t lane, bool active, uint32_t key, const float4 p, int& best) {
   const int tj = __float_as_int(p.w);
   const uint32_t mask = (active && T.hasBim && tj > 0 && tj < kMaxTypes) ? T.partnerMask[tj] : 0u;
   const float maxB2 = mask ? T.maxBindR2[tj] : 0.0f;
   float bestD2 = 3.4e38f;
-  // own bucket, inside the wave
   const uint32_t nLive = *a.dLive;
+  // faces this particle is near
+  const float pc[3] = {p.x, p.y, p.z};
+  int lo[3], hi[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float q = (pc[d] - T.origin[d]) / T.cellSize[d];
+    const float f = q - floorf(q);
+    const float rb = sqrtf(maxB2) / T.cellSize[d] + kFaceMargin;
+    lo[d] = (mask && f < rb) ? 1 : 0;
+    hi[d] = (mask && f > 1.0f - rb) ? 1 : 0;
+  }
+  const uint32_t local = key - T.bucketOff;
+  const uint32_t bx = T.gbits[0], by = T.gbits[1];
+  const uint32_t mx = (1u << bx) - 1u, my = (1u << by) - 1u, mz = (1u << T.gbits[2]) - 1u;
+  const uint32_t gx = local & mx, gy = (local >> bx) & my, gz = local >> (bx + by);
+  const bool rows = mask && !(lo[0] & hi[0]) && !(lo[1] & hi[1]) && !(lo[2] & hi[2]) &&
+                    !(lo[0] && gx == 0u) && !(hi[0] && gx == mx);
+  const uint32_t kLo = rows ? key - (uint32_t)lo[0] : key;
+  const uint32_t kHi = rows ? key + (uint32_t)hi[0] : key;
+  // the key range inside the wave
   for (int o = 1; o < 64; ++o) {
     const int up = (int)lane + o, dn = (int)lane - o;
     const uint32_t kUp = __shfl(key, up & 63, 64);
     const uint32_t kDn = __shfl(key, dn & 63, 64);
-    const bool hitUp = active && up < 64 && kUp == key;
-    const bool hitDn = active && dn >= 0 && kDn == key;
-    if (!__any(hitUp || hitDn)) break;  // runs are contiguous: no lane has a partner further out
+    const bool hitUp = active && up < 64 && kUp <= kHi;
+    const bool hitDn = active && dn >= 0 && kDn >= kLo;
+    if (!__any(hitUp || hitDn)) break;  // keys ascend: no lane has a partner further out
     const float4 qUp = make_float4(__shfl(p.x, up & 63, 64), __shfl(p.y, up & 63, 64), __shfl(p.z, up & 63, 64),
                                    __shfl(p.w, up & 63, 64));
     const float4 qDn = make_float4(__shfl(p.x, dn & 63, 64), __shfl(p.y, dn & 63, 64), __shfl(p.z, dn & 63, 64),
@@ -846,51 +869,34 @@ __device__ __forceinline__ void wave_search(const ReplicaTables& T, const CandAr
   const uint32_t key0 = __shfl(key, 0, 64), key63 = __shfl(key, 63, 64);
   if (!mask) return;
   const uint32_t* __restrict__ first = a.first;
-  // the part of the own bucket before the wave (lanes holding the wave's
-  // first key) or after it (lanes holding its last key): [first[key], wbase)
-  // and [wbase + 64, first[key + 1]), empty when the bucket starts or ends
-  // inside the wave
-  const bool spillLo = key == key0 && wbase > 0;
-  const bool spillHi = key == key63 && wbase + 64 < nLive;
-  // neighbouring buckets across the faces this particle is near
-  const float pc[3] = {p.x, p.y, p.z};
-  int lo[3], hi[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const float q = (pc[d] - T.origin[d]) / T.cellSize[d];
-    const float f = q - floorf(q);
-    const float rb = sqrtf(maxB2) / T.cellSize[d] + kFaceMargin;
-    lo[d] = f < rb ? 1 : 0;
-    hi[d] = f > 1.0f - rb ? 1 : 0;
-  }
-  const uint32_t local = key - T.bucketOff;
-  const uint32_t bx = T.gbits[0], by = T.gbits[1];
-  const uint32_t mx = (1u << bx) - 1u, my = (1u << by) - 1u, mz = (1u << T.gbits[2]) - 1u;
-  const uint32_t gx = local & mx, gy = (local >> bx) & my, gz = local >> (bx + by);
-  if (!(lo[0] & hi[0]) && !(lo[1] & hi[1]) && !(lo[2] & hi[2])) {
-    // at most one neighbour per axis (binding reach below half a cell): the
-    // bucket bounds of the own-bucket spill and of up to 7 neighbours are
-    // loaded together, then the ranges are scanned
-    const uint32_t act = (uint32_t)(lo[0] | hi[0]) | ((uint32_t)(lo[1] | hi[1]) << 1) |
-                         ((uint32_t)(lo[2] | hi[2]) << 2);
-    const uint32_t xs = (gx + (hi[0] ? 1u : mx)) & mx;
+  // the parts of the key range before the wave ([first[kLo], wbase), when
+  // the wave's first key is in the range) and after it ([wbase + 64,
+  // first[kHi + 1]), when its last key is); empty when the range starts or
+  // ends inside the wave
+  const bool spillLo = key0 >= kLo && wbase > 0;
+  const bool spillHi = key63 <= kHi && wbase + 64 < nLive;
+  if (rows) {
+    // at most one other row per axis: every range bound is loaded first,
+    // then the ranges are scanned
+    const uint32_t act = (uint32_t)(lo[1] | hi[1]) | ((uint32_t)(lo[2] | hi[2]) << 1);
     const uint32_t ys = (gy + (hi[1] ? 1u : my)) & my;
     const uint32_t zs = (gz + (hi[2] ? 1u : mz)) & mz;
-    const uint32_t oLo = spillLo ? first[key] : wbase;
-    const uint32_t oHi = spillHi ? first[key + 1] : wbase + 64;
-    uint32_t fa[7], fb[7];
+    const uint32_t xa = gx - (uint32_t)lo[0], xb = gx + (uint32_t)hi[0] + 1u;
+    const uint32_t oLo = spillLo ? first[kLo] : wbase;
+    const uint32_t oHi = spillHi ? first[kHi + 1] : wbase + 64;
+    uint32_t fa[3], fb[3];
 #pragma unroll
-    for (int c = 1; c < 8; ++c) {
+    for (int c = 1; c < 4; ++c) {
       const bool use = ((uint32_t)c & ~act) == 0u;
-      const uint32_t xx = (c & 1) ? xs : gx, yy = (c & 2) ? ys : gy, zz = (c & 4) ? zs : gz;
-      const uint32_t bk = T.bucketOff + (zz << (bx + by)) + (yy << bx) + xx;
-      fa[c - 1] = use ? first[bk] : 0u;
-      fb[c - 1] = use ? first[bk + 1u] : 0u;
+      const uint32_t yy = (c & 1) ? ys : gy, zz = (c & 2) ? zs : gz;
+      const uint32_t row = T.bucketOff + (zz << (bx + by)) + (yy << bx);
+      fa[c - 1] = use ? first[row + xa] : 0u;
+      fb[c - 1] = use ? first[row + xb] : 0u;
     }
     if (spillLo) scan_range(T, a.sortedPT, oLo, wbase, j, p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
     if (spillHi) scan_range(T, a.sortedPT, wbase + 64, oHi, j, p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
 #pragma unroll
-    for (int c = 0; c < 7; ++c)
+    for (int c = 0; c < 3; ++c)
       scan_range(T, a.sortedPT, fa[c], fb[c], j, p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
     return;
   }
