@@ -363,18 +363,24 @@ __global__ __launch_bounds__(256) void k_sort_down(const uint32_t* __restrict__ 
   const uint32_t tile = xcd_tile(blockIdx.x, nTiles);
   const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
 
-  // Global start of each digit for this tile.
+  // Global start of each digit for this tile; thread t owns the DPT
+  // consecutive digits from t * DPT (one block scan whatever the radix).
   {
-    uint32_t carry = 0;
+    uint32_t v[DPT], sum = 0;
 #pragma unroll
     for (int q = 0; q < DPT; ++q) {
-      const uint32_t d = q * 256 + t;
-      const uint32_t v = (d < (uint32_t)RADIX) ? totals[d] : 0u;
-      const uint32_t ex = block_exscan256(v, wsum, &tot);
-      if (d < (uint32_t)RADIX) s_base[d] = carry + ex + counts[(size_t)d * stride + tile];
-      carry += tot;
-      __syncthreads();
+      const uint32_t d = t * DPT + q;
+      v[q] = (d < (uint32_t)RADIX) ? totals[d] : 0u;
+      sum += v[q];
     }
+    uint32_t run = block_exscan256(sum, wsum, &tot);
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+      const uint32_t d = t * DPT + q;
+      if (d < (uint32_t)RADIX) s_base[d] = run + counts[(size_t)d * stride + tile];
+      run += v[q];
+    }
+    __syncthreads();
   }
   for (uint32_t d = t; d < (uint32_t)RADIX; d += 256) {
     s_wcnt[0][d] = 0;
@@ -407,31 +413,31 @@ __global__ __launch_bounds__(256) void k_sort_down(const uint32_t* __restrict__ 
     if (valid && (mm & lt) == 0ull) s_wcnt[w][d] = prev + (uint32_t)__popcll(mm);
   }
   __syncthreads();
-  // Exclusive scan over waves per digit, and over digits for the tile.
+  // Exclusive scan over waves per digit, and over digits for the tile (the
+  // same digit ownership, one block scan).
   {
-    uint32_t carry = 0;
+    uint32_t c[DPT][4], tsum = 0;
 #pragma unroll
     for (int q = 0; q < DPT; ++q) {
-      const uint32_t d = q * 256 + t;
-      uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
-      if (d < (uint32_t)RADIX) {
-        c0 = s_wcnt[0][d];
-        c1 = s_wcnt[1][d];
-        c2 = s_wcnt[2][d];
-        c3 = s_wcnt[3][d];
-      }
-      const uint32_t sum = c0 + c1 + c2 + c3;
-      const uint32_t ex = block_exscan256(sum, wsum, &tot);
+      const uint32_t d = t * DPT + q;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) c[q][k] = (d < (uint32_t)RADIX) ? s_wcnt[k][d] : 0u;
+      tsum += c[q][0] + c[q][1] + c[q][2] + c[q][3];
+    }
+    uint32_t run = block_exscan256(tsum, wsum, &tot);
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+      const uint32_t d = t * DPT + q;
       if (d < (uint32_t)RADIX) {
         s_wcnt[0][d] = 0;
-        s_wcnt[1][d] = c0;
-        s_wcnt[2][d] = c0 + c1;
-        s_wcnt[3][d] = c0 + c1 + c2;
-        s_tstart[d] = carry + ex;
+        s_wcnt[1][d] = c[q][0];
+        s_wcnt[2][d] = c[q][0] + c[q][1];
+        s_wcnt[3][d] = c[q][0] + c[q][1] + c[q][2];
+        s_tstart[d] = run;
       }
-      carry += tot;
-      __syncthreads();
+      run += c[q][0] + c[q][1] + c[q][2] + c[q][3];
     }
+    __syncthreads();
   }
 #pragma unroll
   for (int it = 0; it < ITEMS; ++it) {
