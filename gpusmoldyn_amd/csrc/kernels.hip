@@ -81,7 +81,10 @@ struct ProducerLds {
   int32_t prodY[kMaxRxn];
 };
 
-__global__ __launch_bounds__(kBS) void k_producer(ProducerArgs a) {
+// The replica tables arrive as a separate read-only, non-aliased argument:
+// their wave-uniform fields then load through the scalar cache, whose waits
+// do not hold up the particle records in flight.
+__global__ __launch_bounds__(kBS) void k_producer(ProducerArgs a, const ReplicaTables* __restrict__ tables) {
   __shared__ uint32_t lds_mt[kMtNN * kBS];
   __shared__ float lds_gauss[kGaussDim];
   __shared__ ProducerLds lds_tab[kBS / 64];
@@ -93,7 +96,7 @@ __global__ __launch_bounds__(kBS) void k_producer(ProducerArgs a) {
   const int rw = __builtin_amdgcn_readfirstlane(r);  // a wave never spans two replicas
   ProducerLds& L = lds_tab[threadIdx.x >> 6];
   if (rw >= 0) {
-    const ReplicaTables& T0 = a.tables[rw];
+    const ReplicaTables& T0 = tables[rw];
     if (lane < (uint32_t)kMaxTypes) {
       L.rate[lane] = T0.rate[lane];
       L.uniCount[lane] = T0.uniCount[lane];
@@ -110,7 +113,7 @@ __global__ __launch_bounds__(kBS) void k_producer(ProducerArgs a) {
   uint32_t nDeaths = 0, nBirths = 0;
 
   if (rw >= 0) {
-    const ReplicaTables& T = a.tables[rw];
+    const ReplicaTables& T = tables[rw];
     const uint32_t S = T.numStreams;
     const uint32_t s = g - T.streamOff;
     const uint32_t s0 = (g & ~63u) - T.streamOff;  // stream of lane 0
@@ -123,6 +126,9 @@ __global__ __launch_bounds__(kBS) void k_producer(ProducerArgs a) {
     const bool kill = a.retire != nullptr && a.retire[rw] != 0u;
     Mt m;
     if (laneOK && !kill) mt_load<kBS>(m, &lds_mt[threadIdx.x], a.mtState, a.mtPar, a.Stot, g);
+    // the stream parameters are complete before the loop, so no wait inside
+    // it covers them (such a wait would also cover the records in flight)
+    asm volatile("" ::"v"(m.a), "v"(m.mb), "v"(m.mc), "v"(m.mti1));
     const uint32_t iters = (n > s0) ? (n - s0 + S - 1) / S : 0;
     // The records of the next kPre particles of the stream are requested
     // before the current one is processed (kPre loads in flight per lane).
@@ -1360,7 +1366,7 @@ hipError_t launch_mt_fill(uint32_t* st, const uint32_t* par, uint32_t S, uint32_
 }
 
 hipError_t launch_producer(const ProducerArgs& a, hipStream_t stream) {
-  hipLaunchKernelGGL(k_producer, dim3(cdiv(a.Stot, kBS)), dim3(kBS), 0, stream, a);
+  hipLaunchKernelGGL(k_producer, dim3(cdiv(a.Stot, kBS)), dim3(kBS), 0, stream, a, a.tables);
   return hipGetLastError();
 }
 
