@@ -960,13 +960,15 @@ __global__ __launch_bounds__(256) void k_candidates(CandArgs a) {
 // writing products into the unsorted arrays at the original indices.
 // Stream state is loaded only by threads that actually draw.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBS) void k_resolve(ResolveArgs a) {
+__global__ __launch_bounds__(kBS) void k_resolve(ResolveArgs a, const ReplicaTables* __restrict__ tables) {
   __shared__ uint32_t lds_mt[kMtNN * kBS];
   const uint32_t g = blockIdx.x * kBS + threadIdx.x;
   if (g >= a.Stot) return;
-  const int r = a.streamReplica[g];
+  // a wave never spans two replicas (stream ranges are 64-aligned), so the
+  // table fields load through the scalar cache
+  const int r = __builtin_amdgcn_readfirstlane(a.streamReplica[g]);
   if (r < 0) return;
-  const ReplicaTables& T = a.tables[r];
+  const ReplicaTables& T = tables[r];
   if (!T.hasBim) return;
   const uint32_t S = T.numStreams;
   const uint32_t s = g - T.streamOff;
@@ -976,9 +978,9 @@ __global__ __launch_bounds__(kBS) void k_resolve(ResolveArgs a) {
   Mt m;
   bool loaded = false;
   uint32_t deaths = 0;
-  // candidates of four consecutive particles of the stream are loaded
+  // candidates of eight consecutive particles of the stream are loaded
   // together; the particles are still handled in increasing j
-  constexpr int kAhead = 4;
+  constexpr int kAhead = 8;
   int cq[kAhead];
   for (uint32_t j0 = lo + s; j0 < hi; j0 += kAhead * S) {
 #pragma unroll
@@ -1513,7 +1515,7 @@ hipError_t launch_candidates(const CandArgs& a, uint32_t maxN, hipStream_t strea
 }
 
 hipError_t launch_resolve(const ResolveArgs& a, hipStream_t stream) {
-  hipLaunchKernelGGL(k_resolve, dim3(cdiv(a.Stot, kBS)), dim3(kBS), 0, stream, a);
+  hipLaunchKernelGGL(k_resolve, dim3(cdiv(a.Stot, kBS)), dim3(kBS), 0, stream, a, a.tables);
   return hipGetLastError();
 }
 
