@@ -724,9 +724,22 @@ __global__ __launch_bounds__(256) void k_first(const uint32_t* __restrict__ keys
   for (uint32_t i = t; i < kFirstSpan; i += 256) s_out[first_pad(i)] = 0xFFFFFFFFu;
   __syncthreads();
   const uint32_t lo = s_rng[0], hi = max(s_rng[1], lo);
-  for (uint32_t j = lo + t; j < hi; j += 256) {
-    const uint32_t k = keys[j];
-    if (k >= h0 && k < h1 && (j == lo || keys[j - 1] != k)) s_out[first_pad(k - h0)] = j;
+  // eight keys (and their predecessors) per thread in flight per round
+  constexpr int kMarkPer = 8;
+  for (uint32_t j0 = lo + t; j0 < hi; j0 += 256u * kMarkPer) {
+    uint32_t kk[kMarkPer], kp[kMarkPer];
+#pragma unroll
+    for (int q = 0; q < kMarkPer; ++q) {
+      const uint32_t j = j0 + (uint32_t)q * 256u;
+      kk[q] = j < hi ? keys[j] : 0u;
+      kp[q] = (j < hi && j > lo) ? keys[j - 1] : 0xFFFFFFFFu;
+    }
+#pragma unroll
+    for (int q = 0; q < kMarkPer; ++q) {
+      const uint32_t j = j0 + (uint32_t)q * 256u;
+      const uint32_t k = kk[q];
+      if (j < hi && k >= h0 && k < h1 && (j == lo || kp[q] != k)) s_out[first_pad(k - h0)] = j;
+    }
   }
   __syncthreads();
   // suffix minimum: own 16 entries, then the minimum over later threads
@@ -1182,14 +1195,25 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
   bool al[kItems], bi[kItems];
   uint32_t ra[kItems], rb[kItems];
   float4 pv[kItems];
+  float wv[kItems];
+  int bt[kItems];
   uint32_t ca = 0, cb = 0;
+  // every load of the survivors' data before the first store (a wait for a
+  // load also waits for the stores issued before it)
 #pragma unroll
   for (int it = 0; it < kItems; ++it) {
     const uint32_t e = tile * kCompactTile + w * kWaveSpan + it * 64 + lane;
     const bool v = e < n;
     pv[it] = v ? a.pt[e] : make_float4(0.f, 0.f, 0.f, 0.f);
+    wv[it] = v ? a.w[e] : 0.0f;
+    bt[it] = v ? a.birthType[e] : 0;
+  }
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    const uint32_t e = tile * kCompactTile + w * kWaveSpan + it * 64 + lane;
+    const bool v = e < n;
     al[it] = v && __float_as_int(pv[it].w) > 0;
-    bi[it] = v && a.birthType[e] > 0;
+    bi[it] = v && bt[it] > 0;
     const uint64_t am = __ballot(al[it]);
     const uint64_t bm = __ballot(bi[it]);
     ra[it] = ca + (uint32_t)__popcll(am & lt);
@@ -1220,20 +1244,30 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
     }
     r = lo;
   }
+  // the replica's offsets in registers, reloaded only when the walk moves on
+  uint32_t nOff = a.newOff[r], aBase = a.aliveBase[r], bBase = a.birthBase[r], rAlive = a.repAlive[r];
+  uint32_t nextOff = r + 1 < a.R ? a.repOff[r + 1] : 0xFFFFFFFFu;
 #pragma unroll
   for (int it = 0; it < kItems; ++it) {
     const uint32_t e = tile * kCompactTile + w * kWaveSpan + it * 64 + lane;
     if (!(al[it] || bi[it])) continue;
-    while (r + 1 < a.R && a.repOff[r + 1] <= e) ++r;
+    while (nextOff <= e) {
+      ++r;
+      nOff = a.newOff[r];
+      aBase = a.aliveBase[r];
+      bBase = a.birthBase[r];
+      rAlive = a.repAlive[r];
+      nextOff = r + 1 < a.R ? a.repOff[r + 1] : 0xFFFFFFFFu;
+    }
     if (al[it]) {
-      const uint32_t dst = a.newOff[r] + (pa + ra[it] - a.aliveBase[r]);
+      const uint32_t dst = nOff + (pa + ra[it] - aBase);
       a.ptOut[dst] = pv[it];
-      a.wOut[dst] = a.w[e];
+      a.wOut[dst] = wv[it];
     }
     if (bi[it]) {
-      const uint32_t dst = a.newOff[r] + a.repAlive[r] + (pb + rb[it] - a.birthBase[r]);
+      const uint32_t dst = nOff + rAlive + (pb + rb[it] - bBase);
       const float4 q = a.addedPos[e];
-      a.ptOut[dst] = make_float4(q.x, q.y, q.z, __int_as_float(a.birthType[e]));
+      a.ptOut[dst] = make_float4(q.x, q.y, q.z, __int_as_float(bt[it]));
       a.wOut[dst] = q.w;
     }
   }
