@@ -918,11 +918,19 @@ __device__ __forceinline__ void wave_search(const ReplicaTables& T, const CandAr
       fa[c - 1] = use ? first[row + xa] : 0u;
       fb[c - 1] = use ? first[row + xb] : 0u;
     }
-    if (spillLo) scan_range(T, a.sortedPT, oLo, wbase, j, p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
-    if (spillHi) scan_range(T, a.sortedPT, wbase + 64, oHi, j, p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
+    // the first particle of every range requested together, then the rest
+    uint32_t ra[5] = {spillLo ? oLo : 0u, spillHi ? wbase + 64 : 0u, fa[0], fa[1], fa[2]};
+    const uint32_t rb[5] = {spillLo ? wbase : 0u, spillHi ? oHi : 0u, fb[0], fb[1], fb[2]};
+    float4 q0[5];
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
-      scan_range(T, a.sortedPT, fa[c], fb[c], j, p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
+    for (int c = 0; c < 5; ++c) q0[c] = ra[c] < rb[c] ? a.sortedPT[ra[c]] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      if (ra[c] < rb[c]) {
+        consider(T, q0[c], ra[c], j, p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
+        scan_range(T, a.sortedPT, ra[c] + 1, rb[c], j, p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
+      }
+    }
     return;
   }
   if (spillLo) scan_range(T, a.sortedPT, first[key], wbase, j, p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
