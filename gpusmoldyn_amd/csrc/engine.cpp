@@ -121,8 +121,8 @@ Engine::~Engine() {
   dfree(dInjPos);
   dfree(dInjType);
   dfree(dInjRep);
-  for (auto& ev : ring) (void)hipEventDestroy(ev.ev);
-  if (pinned) (void)hipHostFree(pinned);
+  if (countRing) (void)hipHostFree(countRing);
+  dfree(dStepCount);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -368,30 +368,34 @@ smd_status Engine::finalize(uint32_t capacity) {
   Nub = total;
   knownN = total;
   stepsDone = 0;
-  HIPCHK(hipHostMalloc((void**)&pinned, kRing * sizeof(uint32_t), hipHostMallocDefault));
-  ring.resize(kRing);
-  for (auto& e : ring) HIPCHK(hipEventCreateWithFlags(&e.ev, hipEventDisableTiming));
+  HIPCHK(hipHostMalloc((void**)&countRing, kRing * sizeof(unsigned long long),
+                       hipHostMallocMapped | hipHostMallocCoherent));
+  std::memset(countRing, 0, kRing * sizeof(unsigned long long));
+  HIPCHK(hipHostGetDevicePointer((void**)&dCountRing, countRing, 0));
+  HIPCHK(dalloc(&dStepCount, 1));
+  HIPCHK(hipMemset(dStepCount, 0, sizeof(uint32_t)));
+  lastSeen = 0;
   HIPCHK(hipStreamSynchronize(stream));
   finalized = true;
   return SMD_OK;
 }
 
 // Bound on the particle count after all enqueued steps, refreshed from the
-// asynchronous count readbacks that have completed.
+// latest count the finalize kernel has published (blocking: after the stream
+// has drained, the count after the last step).
 void Engine::refresh_bound(bool block) {
-  while (pendingCount > 0) {
-    Pending& p = ring[pendingHead];
-    if (hipEventQuery(p.ev) != hipSuccess) {
-      if (!block) break;
-      (void)hipEventSynchronize(p.ev);
-    }
-    knownN = pinned[pendingHead];
-    const uint64_t lag = stepsDone - p.step;
+  if (lastSeen >= stepsDone) return;
+  if (block) (void)hipStreamSynchronize(stream);
+  const uint64_t oldest = stepsDone >= kRing ? stepsDone - kRing + 1 : 1;
+  for (uint64_t s = stepsDone; s > lastSeen && s >= oldest; --s) {
+    const unsigned long long v = __atomic_load_n(&countRing[s & (kRing - 1)], __ATOMIC_ACQUIRE);
+    if ((uint32_t)(v >> 32) != (uint32_t)s) continue;
+    knownN = (uint32_t)v;
     uint64_t b = knownN;
-    for (uint64_t k = 0; k < lag && b < (1ull << 40); ++k) b = step_bound(b);
+    for (uint64_t k = 0; k < stepsDone - s && b < (1ull << 40); ++k) b = step_bound(b);
     Nub = std::min<uint64_t>(Nub, b);
-    pendingHead = (pendingHead + 1) % kRing;
-    --pendingCount;
+    lastSeen = s;
+    break;
   }
 }
 
@@ -442,17 +446,6 @@ smd_status Engine::enqueue_step(StageTimer* timer) {
   if (st != SMD_OK) return st;
   ++stepsDone;
   Nub = std::min<uint64_t>(step_bound(Nub), (uint64_t)cap);
-  return enqueue_readback();
-}
-
-// Asynchronous readback of the current total into the pinned ring.
-smd_status Engine::enqueue_readback() {
-  if (pendingCount == kRing) refresh_bound(true);
-  const int slot = (pendingHead + pendingCount) % kRing;
-  HIPCHK(hipMemcpyAsync(&pinned[slot], dN, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-  HIPCHK(hipEventRecord(ring[slot].ev, stream));
-  ring[slot].step = stepsDone;
-  ++pendingCount;
   return SMD_OK;
 }
 
@@ -497,8 +490,6 @@ smd_status Engine::step_graph(int pairs) {
     HIPCHK(hipGraphLaunch(gexec, stream));
     stepsDone += 2;
     Nub = std::min<uint64_t>(step_bound(step_bound(Nub)), (uint64_t)cap);
-    smd_status st = enqueue_readback();
-    if (st != SMD_OK) return st;
   }
   return SMD_OK;
 }
@@ -597,6 +588,9 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
   fa.tileAlivePre = dTileAlivePre;
   fa.tileBirthPre = dTileBirthPre;
   fa.particleSteps = dParticleSteps;
+  fa.stepCount = dStepCount;
+  fa.countRing = dCountRing;
+  fa.ringMask = kRing - 1;
   fa.injCount = dInjCount;
   fa.zCount = anyZero ? dZCount : nullptr;
   fa.zeroBase = dZeroBase;
@@ -881,6 +875,7 @@ smd_status Engine::set_particles(const float* pos4, const int* types, uint32_t n
   HIPCHK(hipMemcpy(dN, &n, sizeof(uint32_t), hipMemcpyHostToDevice));
   Nub = n;
   knownN = n;
+  lastSeen = stepsDone;  // counts published before this call are stale
   return SMD_OK;
 }
 

@@ -1139,6 +1139,12 @@ __global__ __launch_bounds__(1024) void k_finalize(FinalizeArgs a) {
     a.newOff[a.R] = cN;
     *a.dNnext = cN;
     *a.particleSteps += n;
+    // the host reads the particle count after each step from this ring,
+    // tagged with the step number (no copy or event in the stream)
+    const uint32_t st = *a.stepCount + 1u;
+    *a.stepCount = st;
+    __hip_atomic_store(&a.countRing[st & a.ringMask], ((unsigned long long)st << 32) | cN, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
   }
   // compaction tiles, four consecutive tiles per thread
   constexpr uint32_t kTPT = 4;

@@ -84,15 +84,13 @@ class Engine {
   float4* dZPos = nullptr;
   int* dZType = nullptr;
 
-  // asynchronous count readback
-  struct Pending {
-    hipEvent_t ev = nullptr;
-    uint64_t step = 0;
-  };
-  static constexpr int kRing = 8;
-  std::vector<Pending> ring;
-  uint32_t* pinned = nullptr;
-  int pendingHead = 0, pendingCount = 0;
+  // particle count after each step, written by the finalize kernel into
+  // host-visible memory: slot (step & (kRing - 1)) = step << 32 | count
+  static constexpr uint32_t kRing = 64;
+  unsigned long long* countRing = nullptr;   // host pointer
+  unsigned long long* dCountRing = nullptr;  // its device address
+  uint32_t* dStepCount = nullptr;
+  uint64_t lastSeen = 0;  // latest step whose count has been read
 
   // Serving: fixed replica slots filled and emptied between steps.
   smd_status finalize_slots(uint32_t nSlots, uint32_t slotBucketBits, uint32_t slotStreams, uint32_t capacity,
@@ -118,7 +116,6 @@ class Engine {
 
  private:
   smd_status enqueue_kernels(uint32_t maxN, StageTimer* timer);
-  smd_status enqueue_readback();
   smd_status step_graph(int pairs);
   smd_status alloc_particle_buffers(uint32_t c);
   void free_particle_buffers();

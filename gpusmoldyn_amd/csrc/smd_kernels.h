@@ -88,6 +88,9 @@ struct FinalizeArgs {
   uint32_t* tileAlivePre;
   uint32_t* tileBirthPre;
   unsigned long long* particleSteps;  // += particles entering the step
+  uint32_t* stepCount;                // steps completed (incremented here)
+  unsigned long long* countRing;      // host-visible: slot (steps & ringMask) = steps << 32 | particles
+  uint32_t ringMask;
   const uint32_t* zCount;  // per replica: particles created this step (nullptr: none)
   uint32_t* zeroBase;      // per replica: alive + births, where created particles go
   uint32_t* injCount;      // per replica: particles admitted this step (nullptr: none); reset
