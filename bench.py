@@ -116,6 +116,17 @@ def profiled_traffic(cfg, kernel):
     return None, None, None
 
 
+def stage_graph_bytes(e, steps, ps):
+    """Compulsory HBM bytes of one step of the reference's stage graph, each
+    stage reading its inputs and writing its outputs once:
+    N (140 + 16 P) + 12 C + 336 S with P = ceil(K / 4) passes of 4-bit digits
+    over K key bits, C buckets and S streams (N = mean particles per step)."""
+    lay = e.layout()
+    n = ps / max(steps, 1)
+    P = (lay["key_bits"] + 3) // 4
+    return n * (140 + 16 * P) + 12 * lay["total_buckets"] + 336 * lay["total_streams"]
+
+
 def roofline(e, reps, cfg, nprof=5):
     prof = e.profile(nprof)
     lay = e.layout()
@@ -249,6 +260,9 @@ def main():
         "roofline": rl,
         "step_roofline": step,
     }
+    b_step = stage_graph_bytes(e, steps, ps)
+    gbs = b_step / (secs / max(steps, 1)) / 1e9
+    out["step_stage_graph"] = {"bytes": int(b_step), "gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
     if probe:
         out["copy_probe"] = probe
     if args.config == "c4" and not args.no_plain:
