@@ -368,6 +368,17 @@ __global__ __launch_bounds__(256) void k_sort_down(const uint32_t* __restrict__ 
   if (blockIdx.x >= nTiles) return;
   const uint32_t tile = xcd_tile(blockIdx.x, nTiles);
   const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+  // the tile's keys and values are requested first: they arrive while the
+  // digit starts are computed
+  const uint32_t base = tile * TILE + w * (64 * ITEMS);
+  uint32_t kk[ITEMS], vv[ITEMS], rk[ITEMS];
+#pragma unroll
+  for (int it = 0; it < ITEMS; ++it) {
+    const uint32_t e = base + it * 64 + lane;
+    const bool valid = e < n;
+    kk[it] = valid ? keysIn[e] : 0u;
+    vv[it] = identityVals ? e : (valid ? valsIn[e] : 0u);
+  }
 
   // Global start of each digit for this tile; thread t owns the DPT
   // consecutive digits from t * DPT (one block scan whatever the radix).
@@ -396,16 +407,12 @@ __global__ __launch_bounds__(256) void k_sort_down(const uint32_t* __restrict__ 
   }
   __syncthreads();
 
-  const uint32_t base = tile * TILE + w * (64 * ITEMS);
-  uint32_t kk[ITEMS], vv[ITEMS], rk[ITEMS];
   const uint64_t lt = lanemask_lt();
 #pragma unroll
   for (int it = 0; it < ITEMS; ++it) {
     const uint32_t e = base + it * 64 + lane;
     const bool valid = e < n;
-    const uint32_t k = valid ? keysIn[e] : 0u;
-    kk[it] = k;
-    vv[it] = identityVals ? e : (valid ? valsIn[e] : 0u);
+    const uint32_t k = kk[it];
     const uint32_t d = (k >> shift) & (RADIX - 1);
     uint64_t mm = __ballot(valid);
 #pragma unroll
