@@ -124,14 +124,10 @@ __global__ __launch_bounds__(kBS) void k_producer(ProducerArgs a, const ReplicaT
     // A retiring replica (a finished serving job) has all its particles
     // removed this step, without drawing.
     const bool kill = a.retire != nullptr && a.retire[rw] != 0u;
-    Mt m;
-    if (laneOK && !kill) mt_load<kBS>(m, &lds_mt[threadIdx.x], a.mtState, a.mtPar, a.Stot, g);
-    // the stream parameters are complete before the loop, so no wait inside
-    // it covers them (such a wait would also cover the records in flight)
-    asm volatile("" ::"v"(m.a), "v"(m.mb), "v"(m.mc), "v"(m.mti1));
     const uint32_t iters = (n > s0) ? (n - s0 + S - 1) / S : 0;
     // The records of the next kPre particles of the stream are requested
-    // before the current one is processed (kPre loads in flight per lane).
+    // before the current one is processed (kPre loads in flight per lane);
+    // the first ones before the stream state.
     constexpr int kPre = 4;
     float4 pq[kPre];
 #pragma unroll
@@ -139,6 +135,11 @@ __global__ __launch_bounds__(kBS) void k_producer(ProducerArgs a, const ReplicaT
       const uint32_t iq = s + (uint32_t)q * S;
       pq[q] = (laneOK && iq < n) ? a.pt[off + iq] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
+    Mt m;
+    if (laneOK && !kill) mt_load<kBS>(m, &lds_mt[threadIdx.x], a.mtState, a.mtPar, a.Stot, g);
+    // the stream parameters are complete before the loop, so no wait inside
+    // it covers them (such a wait would also cover the records in flight)
+    asm volatile("" ::"v"(m.a), "v"(m.mb), "v"(m.mc), "v"(m.mti1));
     for (uint32_t it = 0; it < iters; ++it) {
       const uint32_t il = s + it * S;
       const bool act = laneOK && il < n;
@@ -280,14 +281,18 @@ __global__ __launch_bounds__(256) void k_sort_up(const uint32_t* __restrict__ ke
   const uint32_t nTiles = (n + TILE - 1) / TILE;
   if (blockIdx.x >= nTiles) return;
   const uint32_t tile = xcd_tile(blockIdx.x, nTiles);
-  for (int d = threadIdx.x; d < RADIX; d += 256) hist[d] = 0;
-  __syncthreads();
   const uint32_t base = tile * TILE;
-#pragma unroll 4
+  uint32_t kk[ITEMS];
+#pragma unroll
   for (int k = 0; k < ITEMS; ++k) {
     const uint32_t e = base + k * 256 + threadIdx.x;
-    if (e < n) atomicAdd(&hist[(keys[e] >> shift) & (RADIX - 1)], 1u);
+    kk[k] = e < n ? keys[e] : 0xFFFFFFFFu;
   }
+  for (int d = threadIdx.x; d < RADIX; d += 256) hist[d] = 0;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k)
+    if (base + k * 256 + threadIdx.x < n) atomicAdd(&hist[(kk[k] >> shift) & (RADIX - 1)], 1u);
   __syncthreads();
   for (int d = threadIdx.x; d < RADIX; d += 256) counts[(size_t)d * stride + tile] = hist[d];
 }
@@ -1242,12 +1247,12 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
     ca += (uint32_t)__popcll(am);
     cb += (uint32_t)__popcll(bm);
   }
+  uint32_t pa = a.tileAlivePre[tile], pb = a.tileBirthPre[tile];
   if (lane == 0) {
     s_wa[w] = ca;
     s_wb[w] = cb;
   }
   __syncthreads();
-  uint32_t pa = a.tileAlivePre[tile], pb = a.tileBirthPre[tile];
   for (uint32_t k = 0; k < w; ++k) {
     pa += s_wa[k];
     pb += s_wb[k];
