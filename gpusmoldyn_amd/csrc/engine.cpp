@@ -537,6 +537,7 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
     sb.gatherSrc = dPT[cur];
     sb.gatherDst = dSortedPT;
     sb.gatherDeadKey = deadKey;
+    sb.sizeHint = (uint32_t)std::min<uint64_t>(knownN, 0xFFFFFFFFu);
     HIPCHK(launch_sort(sb, dN, maxN, keyBits, radixBits, true, stream));
     mark(2);
     HIPCHK(launch_first(sb.keys, dN, maxN, nb, dFirst, dLive, stream));
@@ -995,6 +996,7 @@ smd_status Engine::debug_stage(int stage, float* pos4, int* types, uint32_t* key
     sb.gatherSrc = dPT[cur];
     sb.gatherDst = dSortedPT;
     sb.gatherDeadKey = deadKey;
+    sb.sizeHint = (uint32_t)std::min<uint64_t>(knownN, 0xFFFFFFFFu);
     HIPCHK(launch_sort(sb, dN, maxN, keyBits, radixBits, true, stream));
     HIPCHK(launch_first(sb.keys, dN, maxN, nb, dFirst, dLive, stream));
   }

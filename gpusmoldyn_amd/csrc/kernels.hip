@@ -1518,10 +1518,12 @@ static hipError_t sort3k_pass(SortBuffers& b, const uint32_t* dN, uint32_t maxTi
 
 hipError_t launch_sort_3k(SortBuffers& b, const uint32_t* dN, uint32_t maxN, uint32_t keyBits, int radixBits,
                           bool identityVals, hipStream_t stream) {
-  // 2048-key tiles up to a bound of 12.6M keys, 4096-key tiles beyond
-  // (measured: 1M-4M keys sort 10-20% faster with the smaller tiles, 9M keys
-  // 3% slower)
-  const int items = maxN <= (3u << 22) ? 8 : 16;
+  // 2048-key tiles up to 6M keys, 4096-key tiles beyond (measured: 1M-4M
+  // keys sort 10-20% faster with the smaller tiles, 9M keys 3% slower).  The
+  // expected count decides when the caller gives one (the bound maxN can be
+  // several times the live count); the grid always covers maxN.
+  const uint32_t expect = b.sizeHint ? std::min(b.sizeHint, maxN) : maxN;
+  const int items = expect <= 6000000u ? 8 : 16;
   const uint32_t maxTiles = cdiv(maxN, 256u * items);
   if (maxTiles == 0) return hipSuccess;
   const uint32_t stride = maxTiles + 1;

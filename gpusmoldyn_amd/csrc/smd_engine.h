@@ -76,7 +76,7 @@ class Engine {
   float4* dViewPos = nullptr;  // view() buffers
   int* dViewType = nullptr;
   uint32_t viewCap = 0;
-  SortBuffers sb = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 1, nullptr, nullptr, 0};
+  SortBuffers sb = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 1, nullptr, nullptr, 0, 0};
   uint32_t *dTileAlive = nullptr, *dTileBirth = nullptr, *dTileAlivePre = nullptr, *dTileBirthPre = nullptr;
   uint32_t* dStatus = nullptr;  // sticky kernel error bits
   // order-0 creation: one stream per replica, a per-step creation buffer

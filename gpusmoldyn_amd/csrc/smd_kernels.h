@@ -42,6 +42,7 @@ struct SortBuffers {
   const float4* gatherSrc;
   float4* gatherDst;
   uint32_t gatherDeadKey;
+  uint32_t sizeHint;  // expected key count (tile size choice only), 0: maxN
 };
 
 struct CandArgs {
