@@ -962,7 +962,11 @@ __device__ __forceinline__ void wave_search(const ReplicaTables& T, const CandAr
   }
 }
 
-__global__ __launch_bounds__(256) void k_candidates(CandArgs a) {
+// The bucket offsets and replica tables arrive as read-only non-aliased
+// arguments, so the wave's replica (searched once, for its first key) and the
+// table fields load through the scalar cache.
+__global__ __launch_bounds__(256) void k_candidates(CandArgs a, const uint32_t* __restrict__ bucketStart,
+                                                    const ReplicaTables* __restrict__ tables) {
   const uint32_t nLive = *a.dLive;
   const uint32_t nBlocks = (nLive + 255u) / 256u;
   if (blockIdx.x >= nBlocks) return;
@@ -975,13 +979,17 @@ __global__ __launch_bounds__(256) void k_candidates(CandArgs a) {
   // inactive lanes get a key no live particle has
   const uint32_t key = active ? a.keys[j] : 0xFFFFFFFFu;
   const float4 p = active ? a.sortedPT[j] : make_float4(0.f, 0.f, 0.f, 0.f);
-  const int r = active ? replica_of_bucket(a.bucketStart, a.R, key) : 0;
-  const int rw = __builtin_amdgcn_readfirstlane(r);
+  // replica of the wave's first key, then lanes move forward past later
+  // replica starts (keys ascend through the wave)
+  const int rw = replica_of_bucket(bucketStart, a.R, __builtin_amdgcn_readfirstlane(key));
+  int r = rw;
+  if (active)
+    while (r + 1 < a.R && bucketStart[r + 1] <= key) ++r;
   int best = -1;
   if (!__any(active && r != rw))
-    wave_search(a.tables[rw], a, j, wbase, lane, active, key, p, best);  // one replica: table loads are scalar
+    wave_search(tables[rw], a, j, wbase, lane, active, key, p, best);  // one replica: table loads are scalar
   else
-    wave_search(a.tables[r], a, j, wbase, lane, active, key, p, best);
+    wave_search(tables[r], a, j, wbase, lane, active, key, p, best);
   if (active) a.cand[j] = best;
 }
 
@@ -1572,7 +1580,7 @@ hipError_t launch_gather(const uint32_t* keys, const uint32_t* vals, const float
 
 hipError_t launch_candidates(const CandArgs& a, uint32_t maxN, hipStream_t stream) {
   if (maxN == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_candidates, dim3(cdiv(maxN, 256)), dim3(256), 0, stream, a);
+  hipLaunchKernelGGL(k_candidates, dim3(cdiv(maxN, 256)), dim3(256), 0, stream, a, a.bucketStart, a.tables);
   return hipGetLastError();
 }
 
