@@ -1217,7 +1217,7 @@ __global__ __launch_bounds__(1024) void k_finalize(FinalizeArgs a) {
 // particles_kernel.cu:97-123, done out of place).  Replica r's new segment is
 // [survivors of r in index order, births of r in parent index order].
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
+__global__ __launch_bounds__(256) void k_scatter(ScatterArgs a, const uint32_t* __restrict__ repOff) {
   __shared__ uint32_t s_wa[16], s_wb[16];
   const uint32_t n = *a.dN;
   const uint32_t tile = blockIdx.x;
@@ -1265,22 +1265,23 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
     pa += s_wa[k];
     pb += s_wb[k];
   }
-  // replica of this thread's first element (last r with repOff[r] <= e);
-  // later elements of the thread only move forward through the replicas
+  // replica of the wave's first element (last r with repOff[r] <= e), by a
+  // wave-uniform search through the scalar cache; each lane's elements only
+  // move forward through the replicas from there
   int r = 0;
   {
-    const uint32_t e0 = tile * kCompactTile + w * kWaveSpan + lane;
+    const uint32_t e0 = tile * kCompactTile + w * kWaveSpan;
     int lo = 0, hi = a.R;
     while (hi - lo > 1) {
       const int mid = (lo + hi) >> 1;
-      if (a.repOff[mid] <= e0) lo = mid;
+      if (repOff[mid] <= e0) lo = mid;
       else hi = mid;
     }
     r = lo;
   }
   // the replica's offsets in registers, reloaded only when the walk moves on
   uint32_t nOff = a.newOff[r], aBase = a.aliveBase[r], bBase = a.birthBase[r], rAlive = a.repAlive[r];
-  uint32_t nextOff = r + 1 < a.R ? a.repOff[r + 1] : 0xFFFFFFFFu;
+  uint32_t nextOff = r + 1 < a.R ? repOff[r + 1] : 0xFFFFFFFFu;
 #pragma unroll
   for (int it = 0; it < kItems; ++it) {
     const uint32_t e = tile * kCompactTile + w * kWaveSpan + it * 64 + lane;
@@ -1291,7 +1292,7 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
       aBase = a.aliveBase[r];
       bBase = a.birthBase[r];
       rAlive = a.repAlive[r];
-      nextOff = r + 1 < a.R ? a.repOff[r + 1] : 0xFFFFFFFFu;
+      nextOff = r + 1 < a.R ? repOff[r + 1] : 0xFFFFFFFFu;
     }
     if (al[it]) {
       const uint32_t dst = nOff + (pa + ra[it] - aBase);
@@ -1631,7 +1632,7 @@ hipError_t launch_mt_init_range(uint32_t* st, const uint32_t* seeds, uint32_t St
 
 hipError_t launch_scatter(const ScatterArgs& a, uint32_t maxN, hipStream_t stream) {
   if (maxN == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_scatter, dim3(cdiv(maxN, kCompactTile)), dim3(256), 0, stream, a);
+  hipLaunchKernelGGL(k_scatter, dim3(cdiv(maxN, kCompactTile)), dim3(256), 0, stream, a, a.repOff);
   return hipGetLastError();
 }
 
