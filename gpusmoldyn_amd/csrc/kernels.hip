@@ -892,15 +892,16 @@ __device__ __forceinline__ void wave_search(const ReplicaTables& T, const CandAr
     const int up = (int)lane + o, dn = (int)lane - o;
     const uint32_t kUp = __shfl(key, up & 63, 64);
     const uint32_t kDn = __shfl(key, dn & 63, 64);
-    const bool hitUp = active && up < 64 && kUp <= kHi;
-    const bool hitDn = active && dn >= 0 && kDn >= kLo;
+    // only lanes that have reactive partners keep the loop going
+    const bool hitUp = mask && up < 64 && kUp <= kHi;
+    const bool hitDn = mask && dn >= 0 && kDn >= kLo;
     if (!__any(hitUp || hitDn)) break;  // keys ascend: no lane has a partner further out
     const float4 qUp = make_float4(__shfl(p.x, up & 63, 64), __shfl(p.y, up & 63, 64), __shfl(p.z, up & 63, 64),
                                    __shfl(p.w, up & 63, 64));
     const float4 qDn = make_float4(__shfl(p.x, dn & 63, 64), __shfl(p.y, dn & 63, 64), __shfl(p.z, dn & 63, 64),
                                    __shfl(p.w, dn & 63, 64));
-    if (mask && hitUp) consider(T, qUp, wbase + (uint32_t)up, j, p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
-    if (mask && hitDn) consider(T, qDn, wbase + (uint32_t)dn, j, p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
+    if (hitUp) consider(T, qUp, wbase + (uint32_t)up, j, p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
+    if (hitDn) consider(T, qDn, wbase + (uint32_t)dn, j, p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
   }
   // the wave's first and last keys, read while every lane is still active
   const uint32_t key0 = __shfl(key, 0, 64), key63 = __shfl(key, 63, 64);
