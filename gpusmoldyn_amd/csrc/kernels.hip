@@ -125,36 +125,29 @@ __global__ __launch_bounds__(kBS) void k_producer(ProducerArgs a, const ReplicaT
     // removed this step, without drawing.
     const bool kill = a.retire != nullptr && a.retire[rw] != 0u;
     const uint32_t iters = (n > s0) ? (n - s0 + S - 1) / S : 0;
-    // The records of the next kPre particles of the stream are requested
-    // before the current one is processed (kPre loads in flight per lane);
-    // the first ones before the stream state.
-    constexpr int kPre = 4;
-    float4 pq[kPre];
-#pragma unroll
-    for (int q = 0; q < kPre; ++q) {
-      const uint32_t iq = s + (uint32_t)q * S;
-      pq[q] = (laneOK && iq < n) ? a.pt[off + iq] : make_float4(0.f, 0.f, 0.f, 0.f);
-    }
+    // Records are requested one pair of particles ahead.
+    auto rec = [&](uint32_t it) __attribute__((always_inline)) {
+      const uint32_t iq = s + it * S;
+      return (laneOK && iq < n) ? a.pt[off + iq] : make_float4(0.f, 0.f, 0.f, 0.f);
+    };
+    float4 p0 = rec(0), p1 = rec(1);
     Mt m;
     if (laneOK && !kill) mt_load<kBS>(m, &lds_mt[threadIdx.x], a.mtState, a.mtPar, a.Stot, g);
     // the stream parameters are complete before the loop, so no wait inside
     // it covers them (such a wait would also cover the records in flight)
     asm volatile("" ::"v"(m.a), "v"(m.mb), "v"(m.mc), "v"(m.mti1));
-    for (uint32_t it = 0; it < iters; ++it) {
+    // returns the particle's births << 32 | deaths
+    auto step_one = [&](const float4 pCur, const uint32_t it) __attribute__((always_inline)) -> uint64_t {
+      uint64_t db = 0;
       const uint32_t il = s + it * S;
       const bool act = laneOK && il < n;
       const uint32_t i = off + il;
-      const float4 pCur = pq[0];
-#pragma unroll
-      for (int q = 0; q + 1 < kPre; ++q) pq[q] = pq[q + 1];
-      const uint32_t iNext = il + (uint32_t)kPre * S;
-      pq[kPre - 1] = (laneOK && iNext < n) ? a.pt[off + iNext] : make_float4(0.f, 0.f, 0.f, 0.f);
       bool alive = false, birth = false;
       if (act && kill) {
         a.pt[i] = make_float4(pCur.x, pCur.y, pCur.z, __int_as_float(0));
         a.key[i] = a.deadKey;
         a.birthType[i] = 0;
-        ++nDeaths;
+        db += 1u;
       } else if (act) {
         const float4 p = pCur;
         int t = __float_as_int(pCur.w);
@@ -223,8 +216,7 @@ __global__ __launch_bounds__(kBS) void k_producer(ProducerArgs a, const ReplicaT
         if (bt) a.addedPos[i] = make_float4(bp[0], bp[1], bp[2], bw);
         alive = t > 0;
         birth = bt > 0;
-        nDeaths += alive ? 0u : 1u;
-        nBirths += birth ? 1u : 0u;
+        db += (alive ? 0ull : 1ull) + (birth ? (1ull << 32) : 0ull);
       }
       // Compaction tile counters: lanes hold consecutive particle indices.
       const uint64_t am = __ballot(alive);
@@ -241,7 +233,23 @@ __global__ __launch_bounds__(kBS) void k_producer(ProducerArgs a, const ReplicaT
         if (bLo) atomicAdd(&a.tileBirth[t0], bLo);
         if (bHi) atomicAdd(&a.tileBirth[t0 + 1], bHi);
       }
+      return db;
+    };
+    uint32_t it = 0;
+    uint64_t db = 0;
+    for (; it + 2 <= iters; it += 2) {
+      // this pair's records were requested one pair ago; the next pair's are
+      // requested before this pair's stores, so waiting for them later does
+      // not wait for those stores
+      const float4 c0 = p0, c1 = p1;
+      p0 = rec(it + 2);
+      p1 = rec(it + 3);
+      db += step_one(c0, it);
+      db += step_one(c1, it + 1);
     }
+    if (it < iters) db += step_one(p0, it);
+    nDeaths += (uint32_t)db;
+    nBirths += (uint32_t)(db >> 32);
     if (laneOK && !kill) mt_store<kBS>(m, a.mtState, a.Stot, g);
   }
   nDeaths = wave_sum(nDeaths);
