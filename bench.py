@@ -154,9 +154,14 @@ def roofline(e, reps, cfg, nprof=5):
     return rl, step
 
 
-def cpu_baseline(reps, budget_s=20.0):
+def cpu_baseline(reps, budget_s=20.0, threads=0):
+    """The oracle (C++ -O3) stepping the first steps of the workload for about
+    budget_s seconds, on `threads` OpenMP threads (0: all host cores)."""
     import oracle
 
+    L = oracle.lib()
+    all_threads = L.orc_num_threads()
+    L.orc_set_threads(threads or all_threads)
     spec, pos, types = reps[0]
     sim = oracle.sim_for(smd, spec, pos, types, cap=4 * len(types) + 1024)
     g = smd.gauss_table()
@@ -168,10 +173,12 @@ def cpu_baseline(reps, budget_s=20.0):
         t_used += time.perf_counter() - t0
         done_ps += n
         k += 1
-    return {"value": done_ps / t_used, "unit": "particle-steps/s", "cores": oracle.lib().orc_num_threads(),
+    cores = L.orc_num_threads()
+    L.orc_set_threads(all_threads)
+    return {"value": done_ps / t_used, "unit": "particle-steps/s", "cores": cores,
             "kind": "port",
-            "sample": f"CPU oracle (C++ -O3, OpenMP producer/pair search), first {k} steps of the same "
-                      f"workload ({len(types)} particles), {t_used:.1f} s"}
+            "sample": f"CPU oracle (C++ -O3, OpenMP producer/pair search, {cores} threads), first {k} steps of "
+                      f"the same workload ({len(types)} particles), {t_used:.1f} s"}
 
 
 def serve(args):
@@ -283,6 +290,7 @@ def main():
                                    "plain_seconds": psecs, "batched_seconds": secs}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(reps, args.cpu_budget)
+        out["cpu_baseline_single_thread"] = cpu_baseline(reps, args.cpu_budget / 2, threads=1)
     print(json.dumps(out), flush=True)
 
 

@@ -72,6 +72,7 @@ def lib():
                                      ctypes.POINTER(ctypes.c_uint)]
         L.orc_candidates_bruteforce.argtypes = [ctypes.POINTER(OrcNet), vp, ctypes.c_uint, vp]
         L.orc_num_threads.restype = ctypes.c_int
+        L.orc_set_threads.argtypes = [ctypes.c_int]
         _lib = L
     return _lib
 

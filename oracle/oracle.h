@@ -114,6 +114,7 @@ int orc_step_stage(orc_sim* s, const float* gauss, int stage, uint32_t* keys, ui
 void orc_candidates_bruteforce(const orc_net* net, const float* sorted_pt, unsigned n_live, int* cand);
 
 int orc_num_threads(void);
+void orc_set_threads(int n);  // OpenMP threads of the later calls (n > 0)
 
 #ifdef __cplusplus
 }
