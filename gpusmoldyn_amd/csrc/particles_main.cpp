@@ -5,6 +5,8 @@
 //   smd_particles [-n=N] [-grid=G] [-i=ITERATIONS] [-benchmark]
 //                 [-qatest | -noprompt] [-device=D] [-counts]
 //                 [-out=particles.bin] [-ref=ref_particles.bin]
+//                 [-counts-every=K -counts-out=counts.csv]
+//                 [-traj-every=K -traj=traj.bin]
 //
 // -benchmark runs ITERATIONS updates (default 300) and prints the
 // reference's timing line.  -qatest/-noprompt runs one update (or
@@ -13,6 +15,14 @@
 // when it differs by more than 5.0 (MAX_EPSILON_ERROR, particles.cpp:57); the
 // test fails when more than 30% (THRESHOLD, :58) are wrong.  -counts prints
 // the species histogram after the run.
+//
+// Result files (written when asked for, also with -benchmark or -qatest):
+//   counts.csv  header "step,species_1,...,species_T", then one row after
+//               step 0 (the initial state) and after every K-th update.
+//   traj.bin    frames of the particle state after step 0 and every K-th
+//               update; each frame is int32 step, int32 n, then n records of
+//               float32 x, y, z and int32 type (little-endian, 16 bytes).
+//               The file starts with the 8 bytes "SMDTRJ01".
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -84,11 +94,61 @@ int main(int argc, char** argv) {
   }
   if (iterations <= 0) iterations = 300;
 
+  const int countsEvery = int_value(argc, argv, "counts-every", 0);
+  const int trajEvery = int_value(argc, argv, "traj-every", 0);
+  std::string countsPath = "counts.csv", trajPath = "traj.bin";
+  value(argc, argv, "counts-out", countsPath);
+  value(argc, argv, "traj", trajPath);
+  FILE* countsFile = nullptr;
+  FILE* trajFile = nullptr;
+  if (countsEvery > 0) {
+    countsFile = std::fopen(countsPath.c_str(), "w");
+    if (!countsFile) {
+      std::fprintf(stderr, "cannot write %s\n", countsPath.c_str());
+      return 2;
+    }
+    std::fprintf(countsFile, "step");
+    for (int t = 1; t < ps.getNumTypes(); ++t) std::fprintf(countsFile, ",species_%d", t);
+    std::fprintf(countsFile, "\n");
+  }
+  if (trajEvery > 0) {
+    trajFile = std::fopen(trajPath.c_str(), "wb");
+    if (!trajFile || std::fwrite("SMDTRJ01", 1, 8, trajFile) != 8) {
+      std::fprintf(stderr, "cannot write %s\n", trajPath.c_str());
+      return 2;
+    }
+  }
+  auto record = [&](int step) {
+    if (countsFile && step % countsEvery == 0) {
+      const std::vector<unsigned> c = ps.getSpeciesCounts();
+      std::fprintf(countsFile, "%d", step);
+      for (size_t t = 1; t < c.size(); ++t) std::fprintf(countsFile, ",%u", c[t]);
+      std::fprintf(countsFile, "\n");
+    }
+    if (trajFile && step % trajEvery == 0) {
+      const int32_t n = ps.getNumParticles();
+      const float* pos = ps.getArray(ParticleSystem::POSITION);
+      const int* types = ps.getTypes();
+      const int32_t head[2] = {step, n};
+      std::fwrite(head, sizeof(int32_t), 2, trajFile);
+      for (int32_t k = 0; k < n; ++k) {
+        float rec[4] = {pos[4 * k], pos[4 * k + 1], pos[4 * k + 2], 0.0f};
+        std::memcpy(&rec[3], &types[k], sizeof(int32_t));
+        std::fwrite(rec, sizeof(float), 4, trajFile);
+      }
+    }
+  };
+  record(0);
+
   const auto t0 = std::chrono::steady_clock::now();
   int done = 0;
-  for (; done < iterations; ++done)
+  for (; done < iterations; ++done) {
     if (!ps.update()) break;
+    if (countsFile || trajFile) record(done + 1);
+  }
   threadSync();
+  if (countsFile) std::fclose(countsFile);
+  if (trajFile) std::fclose(trajFile);
   const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   std::printf("%d particles, total time for %d iterations: %0.3f ms\n", ps.getNumParticles(), done, ms);
   if (ps.lastStatus() != 0) std::fprintf(stderr, "update stopped: %s\n", smd_last_error());

@@ -356,3 +356,38 @@ def test_headless_driver_qatest(smd, gpu, tmp_path):
     r2 = subprocess.run([exe, f"-n={n}", f"-i={it}", "-qatest", f"-out={tmp_path / 'again.bin'}", f"-ref={out}"],
                         capture_output=True, text=True, timeout=120)
     assert r2.returncode == 0 and "TEST PASSED!" in r2.stdout, r2.stdout + r2.stderr
+
+
+def test_headless_driver_result_files(smd, gpu, tmp_path):
+    """smd_particles -counts-every / -traj-every: the species-count CSV and the
+    trajectory frames agree with each other and with the final dump."""
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(smd.__file__), "smd_particles")
+    n, it, k = 4096, 12, 4
+    out, csv, traj = tmp_path / "p.bin", tmp_path / "counts.csv", tmp_path / "traj.bin"
+    print("\n[gpu-test] headless driver result files", flush=True)
+    r = subprocess.run([exe, f"-n={n}", f"-i={it}", "-qatest", f"-out={out}", f"-ref={tmp_path / 'none.bin'}",
+                        f"-counts-every={k}", f"-counts-out={csv}", f"-traj-every={k}", f"-traj={traj}"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = csv.read_text().strip().splitlines()
+    assert lines[0].startswith("step,species_1")
+    rows = [list(map(int, ln.split(","))) for ln in lines[1:]]
+    assert [row[0] for row in rows] == list(range(0, it + 1, k))
+    raw = traj.read_bytes()
+    assert raw[:8] == b"SMDTRJ01"
+    off, frames = 8, []
+    while off < len(raw):
+        step, m = np.frombuffer(raw, np.int32, 2, off)
+        rec = np.frombuffer(raw, np.float32, 4 * m, off + 8).reshape(m, 4)
+        frames.append((int(step), rec))
+        off += 8 + 16 * int(m)
+    assert [f[0] for f in frames] == [row[0] for row in rows]
+    for (step, rec), row in zip(frames, rows):
+        types = rec[:, 3].view(np.int32)
+        assert len(types) == sum(row[1:])
+        for t, c in enumerate(row[1:], start=1):
+            assert int(np.count_nonzero(types == t)) == c
+    final = np.fromfile(out, np.float32).reshape(-1, 4)
+    assert np.array_equal(frames[-1][1][:, :3].view(np.uint32), final[:, :3].view(np.uint32))
