@@ -106,6 +106,7 @@ def lib():
         L.smd_engine_get_streams.argtypes = [vp, ctypes.c_int, u32p, ctypes.c_uint]
         L.smd_engine_set_streams.argtypes = [vp, ctypes.c_int, u32p, ctypes.c_uint]
         L.smd_engine_rxn.argtypes = [vp, ctypes.c_int, ctypes.POINTER(RxnDerived)]
+        L.smd_engine_debug_first.argtypes = [vp, u32p, ctypes.c_uint, ctypes.POINTER(ctypes.c_uint)]
         L.smd_engine_debug_stage.argtypes = [vp, ctypes.c_int, f32p, i32p, u32p, u32p, i32p,
                                              ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(ctypes.c_uint)]
         L.smd_engine_layout.argtypes = [vp] + [ctypes.POINTER(ctypes.c_uint)] * 4
@@ -349,6 +350,16 @@ class Engine:
                                             ctypes.byref(nt), ctypes.byref(nl)), "smd_engine_debug_stage")
         return dict(pos=pos, type=typ, keys=keys, vals=vals, cand=cand[: nl.value], n_total=nt.value,
                     n_live=nl.value)
+
+    def debug_first(self):
+        """Cell table after debug_stage(stage >= 2): first[h] for every key h
+        up to the removed-particle key (= total buckets), then the particle
+        count."""
+        n = ctypes.c_uint(0)
+        _check(lib().smd_engine_debug_first(self._h, None, 0, ctypes.byref(n)), "smd_engine_debug_first")
+        out = np.zeros(n.value, np.uint32)
+        _check(lib().smd_engine_debug_first(self._h, _ptr(out), n.value, ctypes.byref(n)), "smd_engine_debug_first")
+        return out
 
     def profile(self, nsteps):
         us = (ctypes.c_double * 16)()

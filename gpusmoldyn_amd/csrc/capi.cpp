@@ -1,5 +1,6 @@
 // C-ABI of the simulation API (include/gpusmoldyn.h).
 #include <dlfcn.h>
+#include <algorithm>
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -293,6 +294,18 @@ smd_status smd_engine_debug_stage(smd_engine* e, int stage, float* pos4, int* ty
                                   uint32_t* vals_sorted, int* cand, uint32_t* n_total, uint32_t* n_live) {
   if (!e || stage < 1 || stage > 4) return SMD_ERR_INVALID;
   return e->impl->debug_stage(stage, pos4, types, keys_sorted, vals_sorted, cand, n_total, n_live);
+}
+
+smd_status smd_engine_debug_first(smd_engine* e, uint32_t* first, uint32_t cap, uint32_t* n_out) {
+  if (!e || !e->impl->finalized) return SMD_ERR_INVALID;
+  const uint32_t words = e->impl->nb + 1;
+  if (n_out) *n_out = words;
+  if (first && cap) {
+    CAPI_HIPCHK(hipSetDevice(e->impl->device));
+    CAPI_HIPCHK(hipStreamSynchronize(e->impl->stream));
+    CAPI_HIPCHK(hipMemcpy(first, e->impl->dFirst, std::min(cap, words) * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  }
+  return SMD_OK;
 }
 
 smd_status smd_engine_layout(smd_engine* e, uint32_t* total_buckets, uint32_t* key_bits, uint32_t* total_streams,

@@ -203,6 +203,13 @@ smd_status smd_engine_rxn(smd_engine* e, int r, smd_rxn_derived* out);
  * 4 = + resolve.  Arrays are sized for the current total particle count. */
 smd_status smd_engine_debug_stage(smd_engine* e, int stage, float* pos4, int* types, uint32_t* keys_sorted,
                                   uint32_t* vals_sorted, int* cand, uint32_t* n_total, uint32_t* n_live);
+/* After smd_engine_debug_stage with stage >= 2: the cell table, first[h] =
+ * lower bound of key h in the sorted keys for h <= total buckets (key total
+ * buckets marks removed particles, so first[total buckets] = live count) and
+ * first[total buckets + 1] = particle count: the cellStart/cellEnd of
+ * particles_kernel.cu:209-272 for every bucket.  cap words at most; *n_out =
+ * total buckets + 2. */
+smd_status smd_engine_debug_first(smd_engine* e, uint32_t* first, uint32_t cap, uint32_t* n_out);
 /* Number of buckets (keys deadKey = total buckets) and key bits. */
 smd_status smd_engine_layout(smd_engine* e, uint32_t* total_buckets, uint32_t* key_bits, uint32_t* total_streams,
                              uint32_t* capacity);

@@ -165,6 +165,26 @@ def _engine(smd, spec, pos, types, radix_bits=0):
 
 
 @pytest.mark.parametrize("name", list(WORKLOADS))
+def test_cell_table(smd, gpu, name):
+    # first[h] = lower bound of key h in the sorted keys (empty buckets
+    # included; key total buckets marks removed particles), then the count
+    spec, pos, types = WORKLOADS[name]()
+    banner(f"cell table workload {name}")
+    e = _engine(smd, spec, pos, types)
+    got = e.debug_stage(2, len(types))
+    first = e.debug_first()
+    lay = e.layout()
+    e.close()
+    keys = got["keys"][: got["n_total"]]
+    dead = lay["total_buckets"]
+    assert len(first) == dead + 2
+    want = np.searchsorted(keys, np.arange(dead + 1, dtype=np.uint64), side="left")
+    assert np.array_equal(first[: dead + 1], want)
+    assert first[dead] == got["n_live"]
+    assert first[dead + 1] == got["n_total"]
+
+
+@pytest.mark.parametrize("name", list(WORKLOADS))
 def test_stage_parity(smd, orc, gpu, name):
     spec, pos, types = WORKLOADS[name]()
     g = smd.gauss_table()
