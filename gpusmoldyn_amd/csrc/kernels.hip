@@ -870,10 +870,27 @@ constexpr float kFaceMargin = 1.0f / 64.0f;
 // wrap along x, or a binding reach above half a cell, take the per-bucket
 // search instead.
 __device__ __forceinline__ void wave_search(const ReplicaTables& T, const CandArgs& a, uint32_t j, uint32_t wbase,
-                                            uint32_t lane, bool active, uint32_t key, const float4 p, int& best) {
+                                            uint32_t lane, bool active, uint32_t key, const float4 p, int& best,
+                                            bool uniform) {
   const int tj = __float_as_int(p.w);
-  const uint32_t mask = (active && T.hasBim && tj > 0 && tj < kMaxTypes) ? T.partnerMask[tj] : 0u;
-  const float maxB2 = mask ? T.maxBindR2[tj] : 0.0f;
+  // one replica in the wave: the per-type entries sit one per lane (lane t
+  // holds type t's) and reach each lane by a shuffle, so no load waits for
+  // the particle's type; otherwise each lane loads its own replica's entry
+  static_assert(kMaxTypes <= 64, "one type per lane");
+  const bool typed = active && T.hasBim && tj > 0 && tj < kMaxTypes;
+  uint32_t pm;
+  float mb;
+  if (uniform) {
+    const uint32_t pmLane = T.partnerMask[lane % kMaxTypes];
+    const float mbLane = T.maxBindR2[lane % kMaxTypes];
+    pm = __shfl(pmLane, typed ? tj : 0, 64);
+    mb = __shfl(mbLane, typed ? tj : 0, 64);
+  } else {
+    pm = typed ? T.partnerMask[tj] : 0u;
+    mb = typed ? T.maxBindR2[tj] : 0.0f;
+  }
+  const uint32_t mask = typed ? pm : 0u;
+  const float maxB2 = mask ? mb : 0.0f;
   float bestD2 = 3.4e38f;
   const uint32_t nLive = *a.dLive;
   // faces this particle is near
@@ -996,9 +1013,9 @@ __global__ __launch_bounds__(256) void k_candidates(CandArgs a, const uint32_t* 
     while (r + 1 < a.R && bucketStart[r + 1] <= key) ++r;
   int best = -1;
   if (!__any(active && r != rw))
-    wave_search(tables[rw], a, j, wbase, lane, active, key, p, best);  // one replica: table loads are scalar
+    wave_search(tables[rw], a, j, wbase, lane, active, key, p, best, true);  // one replica: scalar table loads
   else
-    wave_search(tables[r], a, j, wbase, lane, active, key, p, best);
+    wave_search(tables[r], a, j, wbase, lane, active, key, p, best, false);
   if (active) a.cand[j] = best;
 }
 
