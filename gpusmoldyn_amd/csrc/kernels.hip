@@ -1137,10 +1137,57 @@ __device__ __forceinline__ uint32_t block_exscan1024(uint32_t v, uint32_t* wsum,
   return pre + x - v;
 }
 
+// Exclusive scans of K values per thread over the 1024-thread workgroup in
+// one pass (the K shuffle chains run side by side); every thread also gets
+// the K totals.
+template <int K>
+__device__ __forceinline__ void block_exscan1024_k(const uint32_t (&v)[K], uint32_t (&ex)[K], uint32_t (&tot)[K],
+                                                   uint32_t (*wsum)[16]) {
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  uint32_t x[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) x[k] = v[k];
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const uint32_t y = __shfl_up(x[k], o, 64);
+      if (lane >= (uint32_t)o) x[k] += y;
+    }
+  }
+  if (lane == 63) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) wsum[k][w] = x[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    uint32_t pre = 0, all = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < 16u; ++q) {
+      const uint32_t ws = wsum[k][q];
+      pre += q < w ? ws : 0u;
+      all += ws;
+    }
+    ex[k] = pre + x[k] - v[k];
+    tot[k] = all;
+  }
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(1024) void k_finalize(FinalizeArgs a) {
-  __shared__ uint32_t wsum[16];
-  __shared__ uint32_t tot;
+  __shared__ uint32_t wsum[3][16];
   const uint32_t n = *a.dN;  // read before any store (the arguments may alias)
+  // the first round of compaction-tile counters is requested right away
+  constexpr uint32_t kTPT = 4;  // consecutive tiles per thread
+  const uint32_t nTiles = (n + kCompactTile - 1) / kCompactTile;
+  uint32_t va[kTPT], vb[kTPT];
+#pragma unroll
+  for (uint32_t k = 0; k < kTPT; ++k) {
+    const uint32_t e = threadIdx.x * kTPT + k;
+    va[k] = e < nTiles ? a.tileAlive[e] : 0u;
+    vb[k] = e < nTiles ? a.tileBirth[e] : 0u;
+  }
   // replicas
   uint32_t cN = 0, cA = 0, cB = 0;
   for (uint32_t b = 0; b < (uint32_t)a.R; b += 1024) {
@@ -1153,19 +1200,14 @@ __global__ __launch_bounds__(1024) void k_finalize(FinalizeArgs a) {
       if (a.injCount) inj = a.injCount[r];
     }
     // new segment of replica r: [survivors, births, created, admitted]
-    const uint32_t exN = block_exscan1024(alive + births + zc + inj, wsum, &tot);
-    const uint32_t tN = tot;
-    __syncthreads();
-    const uint32_t exA = block_exscan1024(alive, wsum, &tot);
-    const uint32_t tA = tot;
-    __syncthreads();
-    const uint32_t exB = block_exscan1024(births, wsum, &tot);
-    const uint32_t tB = tot;
+    const uint32_t v[3] = {alive + births + zc + inj, alive, births};
+    uint32_t ex[3], tot[3];
+    block_exscan1024_k<3>(v, ex, tot, wsum);
     if (r < (uint32_t)a.R) {
-      a.newOff[r] = cN + exN;
-      a.newCount[r] = alive + births + zc + inj;
-      a.aliveBase[r] = cA + exA;
-      a.birthBase[r] = cB + exB;
+      a.newOff[r] = cN + ex[0];
+      a.newCount[r] = v[0];
+      a.aliveBase[r] = cA + ex[1];
+      a.birthBase[r] = cB + ex[2];
       a.repAlive[r] = alive;
       a.repDeaths[r] = 0;
       a.repBirths[r] = 0;
@@ -1176,10 +1218,9 @@ __global__ __launch_bounds__(1024) void k_finalize(FinalizeArgs a) {
       }
       if (a.retire) a.retire[r] = 0;
     }
-    cN += tN;
-    cA += tA;
-    cB += tB;
-    __syncthreads();
+    cN += tot[0];
+    cA += tot[1];
+    cB += tot[2];
   }
   if (threadIdx.x == 0) {
     a.newOff[a.R] = cN;
@@ -1192,36 +1233,32 @@ __global__ __launch_bounds__(1024) void k_finalize(FinalizeArgs a) {
     __hip_atomic_store(&a.countRing[st & a.ringMask], ((unsigned long long)st << 32) | cN, __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_SYSTEM);
   }
-  // compaction tiles, four consecutive tiles per thread
-  constexpr uint32_t kTPT = 4;
-  const uint32_t nTiles = (n + kCompactTile - 1) / kCompactTile;
+  // compaction tiles: exclusive prefixes of both counters, counters reset
   uint32_t carA = 0, carB = 0;
   for (uint32_t b = 0; b < nTiles; b += 1024 * kTPT) {
     const uint32_t e0 = b + threadIdx.x * kTPT;
-    uint32_t va[kTPT], vb[kTPT], sa = 0, sbv = 0;
-    // every load before the first store (the counters are reset here)
+    if (b > 0) {
 #pragma unroll
-    for (uint32_t k = 0; k < kTPT; ++k) {
-      const uint32_t e = e0 + k;
-      va[k] = e < nTiles ? a.tileAlive[e] : 0u;
-      vb[k] = e < nTiles ? a.tileBirth[e] : 0u;
-      sa += va[k];
-      sbv += vb[k];
+      for (uint32_t k = 0; k < kTPT; ++k) {
+        const uint32_t e = e0 + k;
+        va[k] = e < nTiles ? a.tileAlive[e] : 0u;
+        vb[k] = e < nTiles ? a.tileBirth[e] : 0u;
+      }
     }
+    uint32_t sum[2] = {0u, 0u};
 #pragma unroll
     for (uint32_t k = 0; k < kTPT; ++k) {
       const uint32_t e = e0 + k;
+      sum[0] += va[k];
+      sum[1] += vb[k];
       if (e < nTiles) {
         a.tileAlive[e] = 0;
         a.tileBirth[e] = 0;
       }
     }
-    const uint32_t exA = block_exscan1024(sa, wsum, &tot);
-    const uint32_t tA = tot;
-    __syncthreads();
-    const uint32_t exB = block_exscan1024(sbv, wsum, &tot);
-    const uint32_t tB = tot;
-    uint32_t ra = carA + exA, rb = carB + exB;
+    uint32_t ex[2], tot[2];
+    block_exscan1024_k<2>(sum, ex, tot, wsum);
+    uint32_t ra = carA + ex[0], rb = carB + ex[1];
 #pragma unroll
     for (uint32_t k = 0; k < kTPT; ++k) {
       const uint32_t e = e0 + k;
@@ -1232,9 +1269,8 @@ __global__ __launch_bounds__(1024) void k_finalize(FinalizeArgs a) {
       ra += va[k];
       rb += vb[k];
     }
-    carA += tA;
-    carB += tB;
-    __syncthreads();
+    carA += tot[0];
+    carB += tot[1];
   }
 }
 
