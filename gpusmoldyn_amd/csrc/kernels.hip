@@ -2,13 +2,14 @@
 //
 // One step over an ensemble of R >= 1 replicas (a plain simulation is R = 1):
 //   k_producer    diffuse + unimolecular reactions + boundaries + hash   (one thread per RNG stream)
-//   k_sort_*      stable LSD radix sort of (bucket key, particle index)
+//   k_sort_*      stable LSD radix sort of (bucket key, particle index); the last digit pass
+//                 also copies each particle's (x, y, z, type) record into sorted order
 //   k_first       lower bound of every bucket in the sorted keys
-//   k_gather      sorted copy of position and type
-//   k_candidates  27-neighbour-cell search for each particle's nearest reactive partner
+//   k_candidates  neighbour-cell search for each particle's nearest reactive partner
 //   k_resolve     mutual-nearest pairs react (bimReact)                (one thread per RNG stream)
 //   k_finalize    per-replica counts, offsets and compaction-tile prefixes (one workgroup)
 //   k_scatter     order-preserving compaction of survivors, births appended per replica
+// (k_gather copies the records after the single-sweep sort variant.)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
