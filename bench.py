@@ -205,7 +205,8 @@ def serve(args):
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"c5: {len(jobs)} jobs, Poisson 20 jobs/s, n log-uniform 1k-256k, steps log-uniform "
-                               "100-10k, 2-4 species, random A+B<->C / decay tables; 32 slots of 2^18 buckets",
+                               "100-10k, 2-4 species, random A+B<->C / decay tables, one stream per 2 particles; "
+                               "32 slots of 2^18 buckets",
                    "jobs": len(jobs), "particle_steps": n_ps},
         "serving": {"p50_latency_s": st["p50_s"], "p99_latency_s": st["p99_s"], "wall_s": st["wall_s"],
                     "jobs": st["jobs"], "last_arrival_s": jobs[-1].arrival},

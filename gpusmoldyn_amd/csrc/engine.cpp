@@ -540,7 +540,7 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
     sb.sizeHint = (uint32_t)std::min<uint64_t>(knownN, 0xFFFFFFFFu);
     HIPCHK(launch_sort(sb, dN, maxN, keyBits, radixBits, true, stream));
     mark(2);
-    HIPCHK(launch_first(sb.keys, dN, maxN, nb, dFirst, dLive, stream));
+    HIPCHK(launch_first(sb.keys, dN, maxN, nb, dFirst, dLive, dTables, slotsMode ? slotBits : 0u, R, stream));
     mark(3);
     CandArgs ca;
     ca.tables = dTables;
@@ -739,7 +739,11 @@ smd_status Engine::finalize_slots(uint32_t nSlots, uint32_t slotBucketBits, uint
   // placeholders own no streams
   std::vector<int> none(Stot, -1);
   HIPCHK(hipMemcpy(dStreamReplica, none.data(), Stot * sizeof(int), hipMemcpyHostToDevice));
-  for (uint32_t r = 0; r < R; ++r) tabs[r].numStreams = 0;
+  // placeholders own no streams and no buckets (their ranges stay reserved)
+  for (uint32_t r = 0; r < R; ++r) {
+    tabs[r].numStreams = 0;
+    tabs[r].numBuckets = 0;
+  }
   HIPCHK(hipMemcpy(dTables, tabs.data(), R * sizeof(ReplicaTables), hipMemcpyHostToDevice));
   for (uint32_t** p : {&dRetire, &dInjCount, &dInjBase, &dInjSrcOff}) {
     HIPCHK(dalloc(p, R));
@@ -998,7 +1002,7 @@ smd_status Engine::debug_stage(int stage, float* pos4, int* types, uint32_t* key
     sb.gatherDeadKey = deadKey;
     sb.sizeHint = (uint32_t)std::min<uint64_t>(knownN, 0xFFFFFFFFu);
     HIPCHK(launch_sort(sb, dN, maxN, keyBits, radixBits, true, stream));
-    HIPCHK(launch_first(sb.keys, dN, maxN, nb, dFirst, dLive, stream));
+    HIPCHK(launch_first(sb.keys, dN, maxN, nb, dFirst, dLive, dTables, slotsMode ? slotBits : 0u, R, stream));
   }
   if (stage >= 3) {
     CandArgs ca;

@@ -725,14 +725,24 @@ __device__ __forceinline__ uint32_t wave_lower_bound(const uint32_t* __restrict_
   return lo + (uint32_t)__popcll(__ballot(less));
 }
 
+// Slot engines give every slot a fixed range of 2^slotBits buckets, of which
+// its job's grid uses the first tables[slot].numBuckets (0 for a slot never
+// used).  No key falls past that and the pair search and resolve read the
+// table up to offset numBuckets, so spans that start beyond it are skipped
+// (slotBits = 0: every span is written).
 __global__ __launch_bounds__(256) void k_first(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ dN,
                                                uint32_t maxN, uint32_t nb, uint32_t* __restrict__ first,
-                                               uint32_t* __restrict__ dLive) {
+                                               uint32_t* __restrict__ dLive, const ReplicaTables* __restrict__ tables,
+                                               uint32_t slotBits, uint32_t nSlots) {
   __shared__ uint32_t s_out[kFirstSpan + kFirstSpan / 32];
   __shared__ uint32_t s_rng[2];
   __shared__ uint32_t s_wmin[4];
-  const uint32_t n = min(*dN, maxN);
   const uint32_t h0 = blockIdx.x * kFirstSpan;
+  if (slotBits) {
+    const uint32_t slot = h0 >> slotBits;
+    if (slot < nSlots && (h0 & ((1u << slotBits) - 1u)) > tables[slot].numBuckets) return;
+  }
+  const uint32_t n = min(*dN, maxN);
   const uint32_t h1 = min(h0 + kFirstSpan, nb);
   const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
   if (w == 0) {
@@ -1630,8 +1640,12 @@ hipError_t launch_sort_3k(SortBuffers& b, const uint32_t* dN, uint32_t maxN, uin
 size_t first_words(uint32_t nb) { return (size_t)nb + 1; }
 
 hipError_t launch_first(const uint32_t* keys, const uint32_t* dN, uint32_t maxN, uint32_t nb, uint32_t* first,
-                        uint32_t* dLive, hipStream_t stream) {
-  hipLaunchKernelGGL(k_first, dim3(cdiv(nb, kFirstSpan)), dim3(256), 0, stream, keys, dN, maxN, nb, first, dLive);
+                        uint32_t* dLive, const ReplicaTables* tables, uint32_t slotBits, uint32_t nSlots,
+                        hipStream_t stream) {
+  // spans must not straddle slots for the skip test
+  if (slotBits && ((1u << slotBits) < kFirstSpan || !tables)) slotBits = 0;
+  hipLaunchKernelGGL(k_first, dim3(cdiv(nb, kFirstSpan)), dim3(256), 0, stream, keys, dN, maxN, nb, first, dLive,
+                     tables, slotBits, nSlots);
   return hipGetLastError();
 }
 

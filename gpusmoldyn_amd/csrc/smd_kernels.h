@@ -148,10 +148,13 @@ int sort_passes(uint32_t keyBits, int radixBits);
 size_t sort_counts_words(uint32_t maxN, int radixBits);
 hipError_t launch_sort(SortBuffers& b, const uint32_t* dN, uint32_t maxN, uint32_t keyBits, int radixBits,
                        bool identityVals, hipStream_t stream);
-// first: first_words(nb) words (first[0..nb]).
+// first: first_words(nb) words (first[0..nb]).  slotBits > 0 (slot engines,
+// slots of 2^slotBits buckets): entries of slot s past offset
+// tables[s].numBuckets are not written.
 size_t first_words(uint32_t nb);
 hipError_t launch_first(const uint32_t* keys, const uint32_t* dN, uint32_t maxN, uint32_t nb, uint32_t* first,
-                        uint32_t* dLive, hipStream_t stream);
+                        uint32_t* dLive, const ReplicaTables* tables, uint32_t slotBits, uint32_t nSlots,
+                        hipStream_t stream);
 hipError_t launch_gather(const uint32_t* keys, const uint32_t* vals, const float4* src, const uint32_t* dN,
                          uint32_t maxN, uint32_t deadKey, float4* sortedPT, hipStream_t stream);
 hipError_t launch_candidates(const CandArgs& a, uint32_t maxN, hipStream_t stream);

@@ -85,15 +85,17 @@ def random_network(rng, n_species):
     return D, rx
 
 
-def make_job(jid, arrival, n, steps, n_species, seed, dt=1e-3, density=2.048):
+def make_job(jid, arrival, n, steps, n_species, seed, dt=1e-3, density=2.048, per_stream=8):
     """A job of n particles at c1's density (2.048 per um^3) in a reflective
-    cube, A and B in equal numbers."""
+    cube, A and B in equal numbers, one random stream per `per_stream`
+    particles (at most 32768 streams)."""
     rng = np.random.default_rng(seed)
     D, rx = random_network(rng, n_species)
     L = (n / density) ** (1.0 / 3.0)
     gb = grid_bits_for(n)
     spec = make_spec(D=D, dt=dt, reactions=rx, box_lo=[0.0] * 3, box_hi=[L] * 3, faces=[BC_REFLECT] * 6,
-                     grid_bits=gb, num_streams=streams_for(n, 8), seed=seed, unbind_offsets=1)
+                     grid_bits=gb, num_streams=min(streams_for(n, per_stream), 32768), seed=seed,
+                     unbind_offsets=1)
     pos = np.empty((n, 4), np.float32)
     pos[:, :3] = rng.uniform(0.0, L, size=(n, 3)).astype(np.float32)
     pos[:, 3] = 1.0
@@ -101,14 +103,18 @@ def make_job(jid, arrival, n, steps, n_species, seed, dt=1e-3, density=2.048):
     return Job(jid, arrival, spec, pos, types, steps, reserve=n)
 
 
-def c5_jobs(n_jobs=200, rate=20.0, nmin=1000, nmax=262144, smin=100, smax=10000, seed=SEED):
-    """Poisson arrivals at `rate` jobs/s; n and steps log-uniform; 2-4 species."""
+def c5_jobs(n_jobs=200, rate=20.0, nmin=1000, nmax=262144, smin=100, smax=10000, seed=SEED, per_stream=2):
+    """Poisson arrivals at `rate` jobs/s; n and steps log-uniform; 2-4 species.
+    A few hundred thousand particles are resident at a time, so each job gets
+    one stream per 2 particles: the producer's parallelism is its stream
+    count, and at one per 8 it ran as few waves as the GPU has SIMDs."""
     rng = np.random.default_rng(seed)
     t = np.cumsum(rng.exponential(1.0 / rate, size=n_jobs))
     ns = np.exp(rng.uniform(math.log(nmin), math.log(nmax), size=n_jobs)).astype(np.int64)
     st = np.exp(rng.uniform(math.log(smin), math.log(smax), size=n_jobs)).astype(np.int64)
     sp = rng.integers(2, 5, size=n_jobs)
-    return [make_job(k, float(t[k]), int(ns[k]), int(st[k]), int(sp[k]), SEED + 1000 + k) for k in range(n_jobs)]
+    return [make_job(k, float(t[k]), int(ns[k]), int(st[k]), int(sp[k]), SEED + 1000 + k, per_stream=per_stream)
+            for k in range(n_jobs)]
 
 
 class Server:

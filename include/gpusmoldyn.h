@@ -207,8 +207,10 @@ smd_status smd_engine_debug_stage(smd_engine* e, int stage, float* pos4, int* ty
  * lower bound of key h in the sorted keys for h <= total buckets (key total
  * buckets marks removed particles, so first[total buckets] = live count) and
  * first[total buckets + 1] = particle count: the cellStart/cellEnd of
- * particles_kernel.cu:209-272 for every bucket.  cap words at most; *n_out =
- * total buckets + 2. */
+ * particles_kernel.cu:209-272 for every bucket.  In a slot engine the
+ * entries of a slot past its job's grid (offset 2^(grid bits) + 1 onward, or
+ * past offset 0 for a slot never used) are not maintained: no key falls there.
+ * cap words at most; *n_out = total buckets + 2. */
 smd_status smd_engine_debug_first(smd_engine* e, uint32_t* first, uint32_t cap, uint32_t* n_out);
 /* Number of buckets (keys deadKey = total buckets) and key bits. */
 smd_status smd_engine_layout(smd_engine* e, uint32_t* total_buckets, uint32_t* key_bits, uint32_t* total_streams,
