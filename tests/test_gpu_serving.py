@@ -30,13 +30,15 @@ def _alone(smd, job):
     return pos, typ
 
 
-@pytest.mark.parametrize("graphs", [True, False])
-def test_job_alone_equals_in_mix(smd, gpu, graphs):
+@pytest.mark.parametrize("graphs,slot_bits", [(True, 12), (False, 12), (True, 14)])
+def test_job_alone_equals_in_mix(smd, gpu, graphs, slot_bits):
+    """slot_bits 14: every job's grid (2^12 buckets at most) leaves most of its
+    slot's range unused, so the cell table skips those spans."""
     from gpusmoldyn_amd import serving
 
     jobs = _jobs()
-    srv = serving.Server(n_slots=4, slot_bucket_bits=12, slot_streams=1024, capacity=1 << 16, inject_cap=1 << 14,
-                         max_chunk=16, graphs=graphs)
+    srv = serving.Server(n_slots=4, slot_bucket_bits=slot_bits, slot_streams=1024, capacity=1 << 16,
+                         inject_cap=1 << 14, max_chunk=16, graphs=graphs)
     results, stats = srv.run(jobs, realtime=False)
     srv.close()
     assert stats["jobs"] == len(jobs)
