@@ -400,8 +400,8 @@ void Engine::refresh_bound(bool block) {
 }
 
 smd_status Engine::grow(uint64_t need) {
-  HIPCHK(hipStreamSynchronize(stream));
   drop_graph();
+  HIPCHK(hipStreamSynchronize(stream));
   uint64_t nc = std::max<uint64_t>(need, 2ull * cap);
   if (nc >= (1ull << 31)) {
     set_error("particle count exceeds the 2^31 capacity limit");
@@ -449,14 +449,44 @@ smd_status Engine::enqueue_step(StageTimer* timer) {
   return SMD_OK;
 }
 
-void Engine::drop_graph() {
-  if (gexec) (void)hipGraphExecDestroy(gexec);
-  gexec = nullptr;
+void Engine::drop_graph(int which) {
+  const int lo = which < 0 ? 0 : which, hi = which < 0 ? 1 : which;
+  bool any = false;
+  for (int p = lo; p <= hi; ++p) any = any || gexec[p];
+  if (!any) return;
+  // A launch of the graph may still be queued or running: the executable
+  // graph (and the kernel arguments it owns) must outlive it.
+  if (stream) {
+    (void)hipSetDevice(device);
+    (void)hipStreamSynchronize(stream);
+  }
+  for (int p = lo; p <= hi; ++p) {
+    if (gexec[p]) (void)hipGraphExecDestroy(gexec[p]);
+    gexec[p] = nullptr;
+    gMaxN[p] = 0;
+  }
+}
+
+Engine::HostSnapshot Engine::snapshot() const {
+  return HostSnapshot{cur, dRepOff, dNewOff, dRepCount, dNewCount, dN, dNnext, sb};
+}
+
+void Engine::restore(const HostSnapshot& s) {
+  cur = s.cur;
+  dRepOff = s.repOff;
+  dNewOff = s.newOff;
+  dRepCount = s.repCount;
+  dNewCount = s.newCount;
+  dN = s.n;
+  dNnext = s.nnext;
+  sb = s.sb;
 }
 
 // Two consecutive steps captured as one hipGraph: after two steps every
 // ping-pong buffer is back in place, so the graph can be replayed as long as
-// the particle bound stays within the grid sizes it was captured with.
+// the particle bound stays within the grid sizes it was captured with.  An
+// eager step (an admission in a slot engine) flips the ping-pong parity, so
+// one graph is kept per parity.
 smd_status Engine::step_graph(int pairs) {
   for (int k = 0; k < pairs; ++k) {
     refresh_bound(false);
@@ -465,29 +495,50 @@ smd_status Engine::step_graph(int pairs) {
       refresh_bound(true);
       need = step_bound(step_bound(Nub));
       if (need > cap) {
-        drop_graph();
-        smd_status st = grow(2 * need);
+        smd_status st = grow(2 * need);  // waits for the stream and drops the graphs
         if (st != SMD_OK) return st;
       }
     }
-    const uint64_t bound = step_bound(Nub);  // input bound of the second step
-    if (gexec && (cur != gParity || bound > gMaxN)) drop_graph();
-    if (!gexec) {
-      const uint32_t maxN = (uint32_t)std::min<uint64_t>(bound, cap);
+    uint64_t bound = step_bound(Nub);  // input bound of the second step
+    const int p = cur;
+    if (gexec[p] && bound > gMaxN[p]) {
+      // a stale count alone must not force a new capture: read the exact one
+      refresh_bound(true);
+      bound = step_bound(Nub);
+    }
+    uint64_t want = bound;
+    if (gexec[p] && bound > gMaxN[p]) {
+      // outgrown: grids with headroom, so that a growing system re-captures rarely
+      want = std::max<uint64_t>(bound, (uint64_t)gMaxN[p] + gMaxN[p] / 2);
+      drop_graph(p);
+    }
+    if (!gexec[p]) {
+      const uint32_t maxN = (uint32_t)std::min<uint64_t>(want, cap);
+      const HostSnapshot before = snapshot();
       hipGraph_t g = nullptr;
       HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
       smd_status st = enqueue_kernels(maxN, nullptr);
       if (st == SMD_OK) st = enqueue_kernels(maxN, nullptr);
       hipError_t e = hipStreamEndCapture(stream, &g);
-      if (st != SMD_OK) return st;
+      // nothing ran: the host-side buffer selection goes back to where it was
+      restore(before);
+      if (st != SMD_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+      }
       HIPCHK(e);
-      e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+      hipGraphExec_t x = nullptr;
+      e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
-      HIPCHK(e);
-      gParity = cur;
-      gMaxN = maxN;
+      if (e != hipSuccess) {
+        if (x) (void)hipGraphExecDestroy(x);
+        HIPCHK(e);
+      }
+      gexec[p] = x;
+      gMaxN[p] = maxN;
     }
-    HIPCHK(hipGraphLaunch(gexec, stream));
+    HIPCHK(hipGraphLaunch(gexec[p], stream));
+    // two steps: every ping-pong selection is back where it was
     stepsDone += 2;
     Nub = std::min<uint64_t>(step_bound(step_bound(Nub)), (uint64_t)cap);
   }

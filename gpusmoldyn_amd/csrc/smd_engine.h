@@ -107,12 +107,14 @@ class Engine {
   std::vector<uint8_t> slotBusy;     // admitted and not yet retired
   std::vector<uint8_t> slotPending;  // admitted, particles not yet entered
 
-  // hipGraph replay of two-step pairs
+  // hipGraph replay of two-step pairs, one executable graph per ping-pong
+  // parity (an eager step between pairs flips the parity)
   bool useGraphs = true;
-  hipGraphExec_t gexec = nullptr;
-  int gParity = 0;
-  uint32_t gMaxN = 0;
-  void drop_graph();
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  uint32_t gMaxN[2] = {0, 0};
+  // Waits for the stream, then destroys the executable graph of parity
+  // `which` (-1: both).
+  void drop_graph(int which = -1);
 
  private:
   smd_status enqueue_kernels(uint32_t maxN, StageTimer* timer);
@@ -126,6 +128,14 @@ class Engine {
   uint64_t step_bound(uint64_t x) const { return (anySplit ? 2 * x : x) + zeroTotal; }
   smd_status check_status();
   smd_status enqueue_step(StageTimer* timer);
+  // Host-side buffer selection that enqueue_kernels advances by one step.
+  struct HostSnapshot {
+    int cur;
+    uint32_t *repOff, *newOff, *repCount, *newCount, *n, *nnext;
+    SortBuffers sb;
+  };
+  HostSnapshot snapshot() const;
+  void restore(const HostSnapshot& s);
 };
 
 }  // namespace smd
