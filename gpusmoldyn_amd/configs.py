@@ -11,6 +11,8 @@ each spec's `seed`.
   (c2, c3 and c4 enter in hash-bucket order, as a lattice initialisation would)
   c4         512 replicas of c1's network, n log-uniform in [2048, 65536],
              kf log-uniform in [1e-4, 1e-2]
+  ref_test1  the reference's Lotka-Volterra scenario (test1)
+  ref_test2  the reference's Michaelis-Menten scenario (test2)
 """
 import math
 
@@ -198,3 +200,50 @@ def c4(n_rep=512, nmin=2048, nmax=65536, kmin=1e-4, kmax=1e-2, per_stream=8, ord
 
 
 C4_STEPS = 1000
+
+
+# The reference's hard-coded scenarios (particleSystem.cpp:319-571), in its
+# world [-1, 1]^3 with the clamp boundary (particles_kernel.cu:60-95),
+# particle radius 1/64 and a 64^3 grid of 1/32 cells (particleSystem.cpp:59-85).
+def _reference_world(D, dt, volume, reactions, unbind_offsets=0, grid_bits=(6, 6, 6), num_streams=32768,
+                     cell=2.0 / 64.0):
+    r = 1.0 / 64.0
+    return make_spec(D=D, dt=dt, reactions=reactions, box_lo=[-1.0] * 3, box_hi=[1.0] * 3,
+                     faces=[BC_CLAMP] * 6, grid_bits=grid_bits, cell_size=[cell] * 3, origin=[-1.0] * 3,
+                     num_streams=num_streams, particle_radius=r, volume=volume, unbind_offsets=unbind_offsets)
+
+
+def ref_test1(n=4096, k2=5e-3, rng_seed=SEED):
+    """Lotka-Volterra (test1Reactions/test1Types, particleSystem.cpp:319-405):
+    A -> 2A at 10/s, A + B -> 2B, B -> 0 at 10/s; D = 0.001, dt = 0.1,
+    volume 10; the first half A, the rest B; products of a split at the
+    reaction point (the reference's unbindRadius = 0).  The reference's
+    k2 = 2000 maps to a binding radius of 8e4 (larger than the world); k2
+    defaults to 5e-3, a radius of 0.21, on an 8^3 grid of 0.25 cells."""
+    spec = _reference_world(
+        D=[0.0, 0.001, 0.001], dt=0.1, volume=10.0,
+        reactions=[dict(order=1, reactants=[A], products=[A, A], rate=10.0),
+                   dict(order=2, reactants=[A, B], products=[B, B], rate=k2),
+                   dict(order=1, reactants=[B], products=[], rate=10.0)],
+        grid_bits=(3, 3, 3), cell=0.25)
+    rng = np.random.default_rng(rng_seed)
+    pos = uniform_box(rng, n, -1.0, 1.0)
+    types = np.where(np.arange(n) < n // 2, A, B).astype(np.int32)
+    return spec, pos, types
+
+
+def ref_test2(n=11000, rng_seed=SEED):
+    """Michaelis-Menten (test2Reactions/test2Types, particleSystem.cpp:408-511):
+    E + S -> ES at 0.01, ES -> S + S at 1 (pgem 0.2; the reference writes
+    products (2, 2) where its comment says E + S), ES -> E + P at 1;
+    D = 0.05 for all four species, dt = 1e-3, volume 90; the first n/11
+    particles E, the rest S."""
+    spec = _reference_world(
+        D=[0.0, 0.05, 0.05, 0.05, 0.05], dt=1e-3, volume=90.0,
+        reactions=[dict(order=2, reactants=[1, 2], products=[3], rate=0.01),
+                   dict(order=1, reactants=[3], products=[2, 2], rate=1.0, pgem=0.2),
+                   dict(order=1, reactants=[3], products=[1, 4], rate=1.0)])
+    rng = np.random.default_rng(rng_seed)
+    pos = uniform_box(rng, n, -1.0, 1.0)
+    types = np.where(np.arange(n) < n // 11, 1, 2).astype(np.int32)
+    return spec, pos, types

@@ -48,7 +48,7 @@ def test_particle_system_class_symbols(smd):
         assert m in out, m
 
 
-def test_struct_layouts(smd):
+def test_struct_layouts(smd, tmp_path):
     # reference layouts: SimParams 104 B (particles_kernel.cuh:46-67),
     # MersenneTwisterState 84 B (MersenneTwister.h:43-47), Reaction 10 pointers
     src = r"""
@@ -62,8 +62,7 @@ int main(void) {
   return 0;
 }
 """
-    tmp = os.path.join(ROOT, "_work", "abi_layout")
-    os.makedirs(os.path.dirname(tmp), exist_ok=True)
+    tmp = str(tmp_path / "abi_layout")
     with open(tmp + ".c", "w") as f:
         f.write(src)
     subprocess.run(["gcc", "-std=c99", "-I", INC, "-o", tmp, tmp + ".c"], check=True)

@@ -1,7 +1,8 @@
 """Statistical checks of the GPU engine against the oracle and against theory.
 
-  * Two-sample KS test (p > 0.01) of species counts: 256 GPU replicas vs 256
-    oracle replicas with independent seeds and initial positions.
+  * Two-sample KS test (p > 0.01) of species-count trajectories on c1's
+    network: 256 GPU replicas vs 256 oracle replicas with independent seeds
+    and initial positions, at four time points.
   * Smoluchowski: A + B -> C measured rate constant equals the requested kf
     within 2% (the binding radius of rxnparam.cpp:bindingradius is the one
     whose steady-state rate is kf, reaction_kernel.cuh applies it).
@@ -31,31 +32,54 @@ def _uniform(rng, n, L):
     return p
 
 
-def test_ks_species_counts_vs_oracle(smd, orc, gpu):
+def test_ks_species_trajectories_vs_oracle(smd, orc, gpu):
+    """c1's network (A + B -> C; D = 1, 1, 0.5; reflective cube at c1's
+    density of 2.048 per um^3), 1024 A + 1024 B per replica and kf = 0.1 so
+    that about 40 reactions happen in 400 steps: the C count at steps 100,
+    200, 300 and 400 of 256 GPU replicas against 256 oracle replicas with
+    independent seeds and positions, two-sample KS p > 0.01 at every point."""
+    from concurrent.futures import ThreadPoolExecutor
+
     from scipy.stats import ks_2samp
 
-    R, nA, nB, L, steps = 256, 300, 300, 10.0, 300
-    rx = [dict(order=2, reactants=[A, B], products=[C], rate=1.0)]
-    types = np.array([A] * nA + [B] * nB, np.int32)
+    R, nA, L, kf, chunk, points = 256, 1024, 10.0, 0.1, 100, 4
+    rx = [dict(order=2, reactants=[A, B], products=[C], rate=kf)]
+    types = np.array([A] * nA + [B] * nA, np.int32)
+    n0 = 2 * nA
+
+    def spec(seed):
+        return _box_spec(smd, rx, L, smd.BC_REFLECT, seed=seed, num_streams=256, grid_bits=(4, 4, 4))
+
     e = smd.Engine(0)
     for k in range(R):
-        spec = _box_spec(smd, rx, L, smd.BC_REFLECT, seed=4357 + k)
-        e.add_replica(spec, _uniform(np.random.default_rng(k), nA + nB, L), types)
+        e.add_replica(spec(4357 + k), _uniform(np.random.default_rng(k), n0, L), types)
     e.finalize()
-    e.step(steps)
-    gpu_events = (nA + nB) - e.counts().astype(np.int64)  # each A + B -> C removes one particle
+    gpu_c = np.empty((points, R), np.int64)
+    for t in range(points):
+        e.step(chunk)
+        gpu_c[t] = n0 - e.counts().astype(np.int64)  # each A + B -> C removes one particle
+    e.close()
 
     g = smd.gauss_table()
-    cpu_events = np.empty(R, np.int64)
-    for k in range(R):
-        spec = _box_spec(smd, rx, L, smd.BC_REFLECT, seed=91000 + k)
-        sim = orc.sim_for(smd, spec, _uniform(np.random.default_rng(50000 + k), nA + nB, L), types)
-        sim.step(g, steps)
-        cpu_events[k] = (nA + nB) - sim.n
-    res = ks_2samp(gpu_events, cpu_events)
-    print(f"KS: GPU mean {gpu_events.mean():.2f}, oracle mean {cpu_events.mean():.2f}, p = {res.pvalue:.3f}")
-    assert gpu_events.mean() > 5
-    assert res.pvalue > 0.01
+    L_orc = orc.lib()
+
+    def oracle_replica(k):
+        L_orc.orc_set_threads(1)  # one replica per worker thread
+        sim = orc.sim_for(smd, spec(91000 + k), _uniform(np.random.default_rng(50000 + k), n0, L), types)
+        out = []
+        for _ in range(points):
+            sim.step(g, chunk)
+            out.append(n0 - sim.n)
+        return out
+
+    with ThreadPoolExecutor(max_workers=16) as ex:
+        cpu_c = np.array(list(ex.map(oracle_replica, range(R))), np.int64).T
+    for t in range(points):
+        res = ks_2samp(gpu_c[t], cpu_c[t])
+        print(f"KS step {(t + 1) * chunk}: GPU mean C {gpu_c[t].mean():.2f}, oracle {cpu_c[t].mean():.2f}, "
+              f"p = {res.pvalue:.3f}", flush=True)
+        assert res.pvalue > 0.01
+    assert gpu_c[-1].mean() > 20
 
 
 def smoluchowski_k_hat(counts_fn, step_fn, nA0, n0, V, steps, skip, dt=1e-3):
