@@ -135,6 +135,16 @@ smd_status smd_stream_seeds(unsigned seed, unsigned count, uint32_t* out) {
   return SMD_OK;
 }
 
+smd_status smd_sfmt_uniform(unsigned seed, unsigned count, float lo, float hi, float* out) {
+  if (count && !out) {
+    set_error("null output");
+    return SMD_ERR_INVALID;
+  }
+  smd::Sfmt g(seed);
+  for (unsigned k = 0; k < count; ++k) out[k] = g.uniform_open(lo, hi);
+  return SMD_OK;
+}
+
 smd_status smd_mt_params(unsigned index, uint32_t* abc) {
   std::string err;
   const uint32_t* dat = smd::mt_dat(err);

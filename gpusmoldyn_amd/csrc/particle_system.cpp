@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "smd_engine.h"
+#include "smd_setup.h"
 
 namespace {
 
@@ -86,6 +87,7 @@ ParticleSystem::ParticleSystem(unsigned int numParticles, smd_u3 gridSize, bool 
   const unsigned bound = 3 * (numParticles / 4);
   for (unsigned i = 0; i < numParticles; ++i) types[i] = i < bound ? 1 : 2;
   if (m_status == 0) m_status = rebuild(pos.data(), types.data(), numParticles, nullptr);
+  fillReactions();
 }
 
 ParticleSystem::ParticleSystem(const smd_sim_spec& spec, const float* pos4, const int* types, unsigned int n,
@@ -93,9 +95,94 @@ ParticleSystem::ParticleSystem(const smd_sim_spec& spec, const float* pos4, cons
     : m_spec(spec), m_device(device) {
   m_params = params_from_spec(m_spec);
   m_status = rebuild(pos4, types, n, nullptr);
+  fillReactions();
 }
 
-ParticleSystem::~ParticleSystem() { smd_engine_destroy(m_engine); }
+ParticleSystem::~ParticleSystem() {
+  smd_engine_destroy(m_engine);
+  delete m_rng;
+}
+
+void ParticleSystem::randPos(smd_f3& pos) {
+  pos.x = m_rng->uniform_open(-1.0f, 1.0f);
+  pos.y = m_rng->uniform_open(-1.0f, 1.0f);
+  pos.z = m_rng->uniform_open(-1.0f, 1.0f);
+}
+
+// reactionList / reactionOrderInfo from the specification and the derived
+// parameters (reaction.h:36-66; fillReactionOrderInfo, smolreact.cpp:173-271:
+// per order, reactions listed by reactant type in global index order, an
+// order-2 reaction under both of its reactants).
+void ParticleSystem::fillReactions() {
+  if (!m_rng) m_rng = new smd::Sfmt(0u);
+  const int R = m_spec.num_reactions, T = m_spec.num_types;
+  m_rOrder.assign(R, 0);
+  m_rNprod.assign(R, 0);
+  m_rBind2.assign(R, 0.0f);
+  m_rUnbind.assign(R, 0.0f);
+  m_rPgem.assign(R, 0.0f);
+  m_rProb.assign(R, 0.0f);
+  m_rRate.assign(R, 0.0f);
+  m_rProdPos.assign(6 * (size_t)R, 0.0f);
+  m_rProducts.assign(R, smd_i2{-1, -1});
+  m_rReactants.assign(R, smd_i2{-1, -1});
+  for (int r = 0; r < R; ++r) {
+    const smd_reaction& x = m_spec.rxn[r];
+    m_rOrder[r] = x.order;
+    m_rNprod[r] = x.nprod;
+    m_rRate[r] = x.rate;
+    m_rReactants[r] = smd_i2{x.order >= 1 ? x.reactants[0] : -1, x.order == 2 ? x.reactants[1] : -1};
+    m_rProducts[r] = smd_i2{x.nprod >= 1 ? x.products[0] : -1, x.nprod >= 2 ? x.products[1] : -1};
+    m_rPgem[r] = x.pgem > 0.0f ? x.pgem : 0.2f;
+    if ((size_t)r < m_rxn.size()) {
+      m_rBind2[r] = m_rxn[r].bind_radius2;
+      m_rUnbind[r] = m_rxn[r].unbind_radius;
+      m_rProb[r] = m_rxn[r].prob;
+      for (int q = 0; q < 6; ++q) m_rProdPos[6 * (size_t)r + q] = m_rxn[r].product_pos[q];
+    }
+  }
+  m_reactionList.order = m_rOrder.data();
+  m_reactionList.bindRadiusSquared = m_rBind2.data();
+  m_reactionList.unbindRadius = m_rUnbind.data();
+  m_reactionList.geminateProbability = m_rPgem.data();
+  m_reactionList.prob = m_rProb.data();
+  m_reactionList.rate = m_rRate.data();
+  m_reactionList.nprod = m_rNprod.data();
+  m_reactionList.products = m_rProducts.data();
+  m_reactionList.reactants = m_rReactants.data();
+  m_reactionList.productPositions = m_rProdPos.data();
+  reactionList = &m_reactionList;
+  for (int o = 0; o < 3; ++o) {
+    m_oCount[o].assign(T, 0);
+    m_oStart[o].assign(T, 0);
+    m_oIndexes[o].clear();
+    for (int r = 0; r < R; ++r) {
+      const smd_reaction& x = m_spec.rxn[r];
+      if (x.order != o) continue;
+      m_oIndexes[o].push_back(r);
+      if (o >= 1) m_oCount[o][x.reactants[0]]++;
+      if (o == 2) m_oCount[o][x.reactants[1]]++;
+    }
+    for (int t = 1; t < T; ++t) m_oStart[o][t] = m_oStart[o][t - 1] + m_oCount[o][t - 1];
+    const int total = T ? m_oStart[o][T - 1] + m_oCount[o][T - 1] : 0;
+    m_oList[o].assign(total, 0);
+    std::vector<int> fill = m_oStart[o];
+    for (int r = 0; r < R && o >= 1; ++r) {
+      const smd_reaction& x = m_spec.rxn[r];
+      if (x.order != o) continue;
+      m_oList[o][fill[x.reactants[0]]++] = r;
+      if (o == 2) m_oList[o][fill[x.reactants[1]]++] = r;
+    }
+    ReactionOrderInfo& info = reactionOrderInfo[o];
+    info.order = o;
+    info.reactionsPerOrder = (int)m_oIndexes[o].size();
+    info.reactionsPerTypeIdx = m_oStart[o].data();
+    info.localToGlobal = m_oList[o].data();
+    info.localToGlobalSize = total;
+    info.numberOfReactionsPerType = m_oCount[o].data();
+    info.reactionIndexes = m_oIndexes[o].data();
+  }
+}
 
 int ParticleSystem::rebuild(const float* pos4, const int* types, unsigned int n, const uint32_t* streams) {
   smd_engine* e = smd_engine_create(m_device, 0);
@@ -275,6 +362,7 @@ int ParticleSystem::setParams(const SimParams& p) {
     return m_status = rc;
   }
   m_params = p;
+  fillReactions();
   return m_status = SMD_OK;
 }
 

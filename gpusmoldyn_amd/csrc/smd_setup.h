@@ -42,6 +42,21 @@ struct HostMt {
 
 bool stream_seeds(uint32_t seed, uint32_t count, uint32_t* out, std::string& err);
 
+// SFMT-19937 (sfmt.cpp): the host generator behind the reference's
+// init_gen_rand / genrand_real3 / unirandOOF (stats.h:35-37).
+struct Sfmt {
+  static constexpr int kWords128 = 156, kWords32 = 4 * kWords128;
+  uint32_t st[kWords128][4];
+  int idx = kWords32;
+  explicit Sfmt(uint32_t s = 0) { seed(s); }
+  void seed(uint32_t s);        // init_gen_rand
+  uint32_t next32();            // gen_rand32
+  double real_open();           // genrand_real3: (0, 1)
+  float uniform_open(float lo, float hi);  // unirandOOF
+ private:
+  void refill();
+};
+
 void set_error(const std::string& msg);
 
 }  // namespace smd

@@ -153,6 +153,10 @@ smd_status smd_spec_cells(const smd_sim_spec* spec, float* cell3, float* origin3
 smd_status smd_stream_seeds(unsigned seed, unsigned count, uint32_t* out);
 /* The dcmt parameters (matrix_a, mask_b, mask_c) of record `index`. */
 smd_status smd_mt_params(unsigned index, uint32_t* abc);
+/* `count` draws of unirandOOF(lo, hi) (stats.h:35-37) from the SFMT-19937
+ * generator after init_gen_rand(seed): the sequence behind
+ * ParticleSystem::randPos (particleSystem.h:115-120). */
+smd_status smd_sfmt_uniform(unsigned seed, unsigned count, float lo, float hi, float* out);
 
 /* ---- simulation engine ---- */
 

@@ -31,13 +31,41 @@
 #include <stdbool.h>
 #endif
 
+/* float3 / uint3 / int2 of the reference's signatures.  In C++ the first two
+ * convert to any vector type with x, y, z members (HIP's float3 and uint3
+ * among them), so code written against the reference's getters compiles
+ * unchanged. */
+typedef struct smd_f3 {
+  float x, y, z;
+#ifdef __cplusplus
+  template <typename V, typename = decltype(((V*)0)->x + ((V*)0)->y + ((V*)0)->z)>
+  operator V() const {
+    V v;
+    v.x = x;
+    v.y = y;
+    v.z = z;
+    return v;
+  }
+#endif
+} smd_f3;
+typedef struct smd_u3 {
+  unsigned int x, y, z;
+#ifdef __cplusplus
+  template <typename V, typename = decltype(((V*)0)->x + ((V*)0)->y + ((V*)0)->z)>
+  operator V() const {
+    V v;
+    v.x = x;
+    v.y = y;
+    v.z = z;
+    return v;
+  }
+#endif
+} smd_u3;
+typedef struct smd_i2 { int x, y; } smd_i2;
+
 #ifdef __cplusplus
 extern "C" {
 #endif
-
-typedef struct smd_f3 { float x, y, z; } smd_f3;
-typedef struct smd_u3 { unsigned int x, y, z; } smd_u3;
-typedef struct smd_i2 { int x, y; } smd_i2;
 
 /* particles_kernel.cuh:46-67 */
 typedef struct SimParams {
@@ -80,6 +108,18 @@ typedef struct Reaction {
   smd_i2* reactants;
   float* productPositions;
 } Reaction;
+
+/* reaction.h:52-66: the per-order reaction superstructure that
+ * fillReactionOrderInfo (smolreact.cpp:173-271) builds (host arrays). */
+typedef struct ReactionOrderInfo {
+  int order;
+  int reactionsPerOrder;
+  int* reactionsPerTypeIdx;       /* where reactant type t starts in localToGlobal */
+  int* localToGlobal;             /* reactions listed per reactant type (global ids) */
+  int localToGlobalSize;
+  int* numberOfReactionsPerType;  /* reactions per reactant type */
+  int* reactionIndexes;           /* global ids of this order's reactions */
+} ReactionOrderInfo;
 
 int cudaInit(int argc, char** argv);
 int cudaGLInit(int argc, char** argv);

@@ -10,11 +10,13 @@
 //   bindingradius                rxnparam.cpp:253-277
 //   unbindingradius              rxnparam.cpp:287-308
 //   rdfsteadystate               rxnparam.cpp:451-475
+//   init_gen_rand / unirandOOF   SFMT/SFMT.cpp, stats.h:35-37 (randPos)
 #include <cstddef>
 #include <cstdint>
 
 #include "rnd/dci.h"
 #include "rxnparam.h"
+#include "stats.h"
 
 void createGaussTable(float* gaussianLookupTable, size_t gaussianTableDim);
 
@@ -56,6 +58,11 @@ double smdref_bindingradius(double rate, double dt, double difc, double b, int r
 }
 double smdref_unbindingradius(double pgem, double dt, double difc, double a) {
   return unbindingradius(pgem, dt, difc, a);
+}
+
+void smdref_sfmt_uniform(uint32_t seed, int count, float lo, float hi, float* out) {
+  init_gen_rand(seed);
+  for (int k = 0; k < count; ++k) out[k] = unirandOOF(lo, hi);
 }
 
 double smdref_rdf_steady_state(double* r, double* rdfa, double* rdfd, int n, double step, double b, double eps) {

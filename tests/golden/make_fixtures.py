@@ -9,9 +9,11 @@ from rnd/genmtrand.cpp, stats.cpp, SFMT/SFMT.cpp and rxnparam.cpp).
                    over a parameter grid that includes every workload value
   rdf.npz          rdfsteadystate (irreversible and reversible, b < 1 and
                    b > 1) on small radial grids: flux and both output rdfs
+  sfmt_uniform.npz unirandOOF(-1, 1) after init_gen_rand(seed), seeds 0 and
+                   4357: the randPos sequence (SFMT-19937)
 
 Run here (the reference tree is not present on GPU machines):
-    make -C oracle ref && python tests/golden/make_fixtures.py
+    make -C oracle ref && python tests/golden/make_fixtures.py [only-name ...]
 """
 import ctypes
 import itertools
@@ -33,8 +35,24 @@ def mt_seed(k):
     return (4357 + 1000 * k) & 0xFFFFFFFF
 
 
+def write_sfmt(L):
+    L.smdref_sfmt_uniform.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                      ctypes.c_void_p]
+    seeds = np.array([0, 4357], np.uint32)
+    # 3 x 2000 draws crosses the 624-word refill boundary several times
+    out = np.zeros((len(seeds), 6000), np.float32)
+    for k, sd in enumerate(seeds):
+        L.smdref_sfmt_uniform(int(sd), out.shape[1], -1.0, 1.0, out[k].ctypes.data_as(ctypes.c_void_p))
+    np.savez_compressed(os.path.join(HERE, "sfmt_uniform.npz"), seeds=seeds, lo=np.float32(-1.0),
+                        hi=np.float32(1.0), values=out)
+
+
 def main():
     L = ctypes.CDLL(REF)
+    if len(sys.argv) > 1:
+        for name in sys.argv[1:]:
+            {"sfmt": write_sfmt, "rdf": write_rdf}[name](L)
+        return 0
     dbl = ctypes.c_double
     L.smdref_mt_stream.argtypes = [ctypes.c_uint32] * 4 + [ctypes.c_int, ctypes.c_void_p]
     L.smdref_gauss_table.argtypes = [ctypes.c_void_p, ctypes.c_int]
@@ -83,6 +101,7 @@ def main():
                         bindingradius=np.array(br), bindingradius_rel=np.array(brr),
                         unbindingradius=np.array(ub))
     write_rdf(L)
+    write_sfmt(L)
     print("fixtures written to", HERE)
 
 

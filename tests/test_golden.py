@@ -41,6 +41,23 @@ def test_mt_params_are_the_dat_records(smd):
     assert np.array_equal(dat[:, 0] & 0xFFFF, np.arange(32768) & 0xFFFF)
 
 
+def test_sfmt_uniform_matches_reference(smd):
+    # randPos's sequence (particleSystem.h:115-120): unirandOOF after
+    # init_gen_rand, recorded from the reference's compiled SFMT.cpp + stats.h
+    f = np.load(os.path.join(GOLD, "sfmt_uniform.npz"))
+    L = smd.lib()
+    for k, seed in enumerate(f["seeds"]):
+        n = f["values"].shape[1]
+        out = np.zeros(n, np.float32)
+        rc = L.smd_sfmt_uniform(ctypes.c_uint(int(seed)), ctypes.c_uint(n), ctypes.c_float(float(f["lo"])),
+                                ctypes.c_float(float(f["hi"])), out.ctypes.data_as(ctypes.c_void_p))
+        assert rc == 0
+        assert np.array_equal(out.view(np.uint32), f["values"][k].view(np.uint32)), f"seed {seed}"
+    # open interval, uniform
+    v = f["values"].ravel()
+    assert v.min() > -1.0 and v.max() < 1.0 and abs(v.mean()) < 4 * (2 / math.sqrt(12)) / math.sqrt(len(v))
+
+
 def test_gauss_table_bitexact(smd):
     ref = np.load(os.path.join(GOLD, "gauss_table.npy"))
     got = smd.gauss_table()

@@ -257,10 +257,18 @@ def test_legacy_scan_and_sort(smd, gpu):
         L.freeArray(dp)
 
 
-def test_particle_system_reference_ctor(smd, gpu):
+def _reference_spec():
+    # the reference constructor's scenario (test3Types/test3Reactions,
+    # particleSystem.cpp:513-571) in its world (particleSystem.cpp:59-85)
+    return configs.make_spec(D=[0.0, 0.3, 0.1], dt=0.01, reactions=[], box_lo=[-1.0] * 3, box_hi=[1.0] * 3,
+                             faces=[BC_CLAMP] * 6, grid_bits=(6, 6, 6), cell_size=[1 / 32.0] * 3,
+                             origin=[-1.0] * 3, num_streams=32768, particle_radius=1 / 64.0, volume=90.0)
+
+
+def test_particle_system_reference_ctor(smd, orc, gpu):
     """ParticleSystem(numParticles, gridSize, false): the reference's active
-    scenario (test3), CONFIG_GRID reset, then update() -- against the engine
-    started from the same state."""
+    scenario (test3), CONFIG_GRID reset, then update() -- against the CPU
+    oracle started from the same state."""
     L = smd.lib()
     L.smd_ps_create_reference.restype = vp
     L.smd_ps_create_reference.argtypes = [ctypes.c_uint] * 4
@@ -288,19 +296,14 @@ def test_particle_system_reference_ctor(smd, gpu):
     want = lat * np.float32(2.0 / (s - 2)) + r - np.float32(1.0)
     assert np.all(p0[:, 3] == 1.0)
     assert np.abs(p0[:, :3] - want).max() <= r * np.float32(0.01) * 1.001
-    spec = configs.make_spec(D=[0.0, 0.3, 0.1], dt=0.01, reactions=[], box_lo=[-1.0] * 3, box_hi=[1.0] * 3,
-                             faces=[BC_CLAMP] * 6, grid_bits=(6, 6, 6), cell_size=[1 / 32.0] * 3,
-                             origin=[-1.0] * 3, num_streams=32768, particle_radius=1 / 64.0, volume=90.0)
-    e = smd.Engine(0)
-    e.add_replica(spec, p0, t0)
-    e.finalize()
+    sim = orc.sim_for(smd, _reference_spec(), p0, t0)
     for _ in range(20):
         assert L.smd_ps_update(ps) == 1
-    e.step(20)
+    sim.step(smd.gauss_table(), 20)
     p1 = np.zeros((n, 4), np.float32)
     L.smd_ps_get_positions(ps, p1.ctypes.data_as(vp), n)
-    ep, et = e.get(0)
-    assert np.array_equal(p1.view(np.uint32), ep.view(np.uint32))
+    rp, rt = sim.state()
+    assert np.array_equal(p1.view(np.uint32), rp.view(np.uint32))
     r = np.float32(1 / 64.0)
     assert p1[:, :3].min() >= np.float32(-1.0) + r and p1[:, :3].max() <= np.float32(1.0) - r
     L.smd_ps_destroy(ps)
@@ -322,11 +325,12 @@ def test_particle_system_update_false_on_extinction(smd, gpu):
     L.smd_ps_destroy(ps)
 
 
-def test_headless_driver_qatest(smd, gpu, tmp_path):
+def test_headless_driver_qatest(smd, orc, gpu, tmp_path):
     """smd_particles -qatest (particles.cpp benchmark/QA mode without GL):
-    the dumped positions equal ParticleSystem(n, 64^3) + CONFIG_GRID reset +
-    the same number of update() calls, and a second run compares PASSED
-    against the first dump."""
+    the dumped positions equal the CPU oracle stepping the reference
+    constructor's CONFIG_GRID state (ParticleSystem(n, 64^3), test3) for the
+    same number of steps, and a second run compares PASSED against the first
+    dump."""
     import subprocess
 
     exe = os.path.join(os.path.dirname(smd.__file__), "smd_particles")
@@ -345,13 +349,17 @@ def test_headless_driver_qatest(smd, gpu, tmp_path):
     for f in ("smd_ps_update", "smd_ps_reset", "smd_ps_destroy"):
         getattr(L, f).argtypes = [vp] + ([ctypes.c_int] if f == "smd_ps_reset" else [])
     L.smd_ps_get_positions.argtypes = [vp, vp, ctypes.c_uint]
+    L.smd_ps_get_types.argtypes = [vp, vp, ctypes.c_uint]
     ps = L.smd_ps_create_reference(n, 64, 64, 64)
     assert L.smd_ps_reset(ps, 1) == 0
-    for _ in range(it):
-        assert L.smd_ps_update(ps) == 1
-    want = np.zeros((n, 4), np.float32)
-    L.smd_ps_get_positions(ps, want.ctypes.data_as(vp), n)
+    p0 = np.zeros((n, 4), np.float32)
+    t0 = np.zeros(n, np.int32)
+    L.smd_ps_get_positions(ps, p0.ctypes.data_as(vp), n)
+    L.smd_ps_get_types(ps, t0.ctypes.data_as(vp), n)
     L.smd_ps_destroy(ps)
+    sim = orc.sim_for(smd, _reference_spec(), p0, t0)
+    sim.step(smd.gauss_table(), it)
+    want, _ = sim.state()
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
     r2 = subprocess.run([exe, f"-n={n}", f"-i={it}", "-qatest", f"-out={tmp_path / 'again.bin'}", f"-ref={out}"],
                         capture_output=True, text=True, timeout=120)
