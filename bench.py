@@ -1,20 +1,32 @@
 """Benchmark of the gpusmoldyn timestep on one MI355X.
 
-  python bench.py [--steps K] [--warmup W] [--config c2|c3|c4|c1|c5]
+  python bench.py [--steps K] [--warmup W] [--config all|c1|c2|c3|c4|c5]
 
 A step is one full timestep (diffuse + unimolecular reactions + boundaries +
 hash + sort + cell table + pair search + bimolecular reactions + compaction)
 of every particle of the workload, with the particle state resident in HBM.
-`value` is particle-steps/s = (sum over timed steps of the particles entering
-the step) / wall time.  Prints one JSON line.
+particle-steps/s = (sum over timed steps of the particles entering the step)
+/ wall time.  Prints one JSON line.
 
-c5 is the serving workload: jobs arriving as a Poisson stream are admitted
-into replica slots and stepped together; `value` is then the aggregate
-particle-steps/s of the whole run and `serving` holds p50/p99 job latency.
+The default (`--config all`) measures what the headline metric names
+("batched vs plain; p99 job latency"):
+  value            c4, the 512-replica batched ensemble (two-step hipGraphs),
+                   particle-steps/s over K timed steps after W warm-up steps;
+  batched_vs_plain the same 512 replicas run one simulation per launch
+                   sequence, eager and graph-captured, K steps each;
+  roofline         the dominant single kernel of the c4 step;
+  c3_4M            the 4,194,304-particle clustered step (the 60%-of-peak
+                   target is stated on it): its stage-graph fraction of peak;
+  serving          c5, the Poisson job stream through the slot engine: p50 and
+                   p99 job latency and aggregate particle-steps/s;
+  cpu_baseline     the CPU oracle on a bounded sample of c4.
+`--config cN` runs one workload alone (profiling runs use these).
 """
 import argparse
 import json
 import os
+import platform
+import subprocess
 import sys
 import time
 
@@ -28,6 +40,10 @@ from gpusmoldyn_amd import configs  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "particle-steps/sec (diffuse+sort+react) 1 GPU, batched vs plain; p99 job latency"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def stage_bytes(name, n, S, nb, passes):
@@ -69,8 +85,9 @@ def build(cfg):
     elif cfg == "c4":
         reps = configs.c4()
         steps = configs.C4_STEPS
-        desc = dict(workload="c4: 512 replicas of c1, n log-uniform 2k-64k, kf log-uniform 1e-4..1e-2",
-                    replicas=len(reps), particles=int(sum(len(r[2]) for r in reps)))
+        desc = dict(workload="c4: 512 replicas of c1's network (A+B->C, reflective cube at 2.048 per um^3), "
+                             "n log-uniform 2k-64k, kf log-uniform 1e-4..1e-2, one stream per 8 particles",
+                    replicas=len(reps), particles=int(sum(len(r[2]) for r in reps)), parallelism="one GPU")
     else:
         raise SystemExit(f"unknown config {cfg}")
     return reps, steps, desc
@@ -98,13 +115,14 @@ def timed(e, steps):
 # stages that are one kernel each, and that kernel's name in the profiles
 SINGLE_KERNEL = {"producer": "k_producer", "first": "k_first", "candidates": "k_candidates", "resolve": "k_resolve",
                  "scatter": "k_scatter"}
-PROFILE = {"c2": "step_kernels_c2_1M.md", "c3": "step_kernels_c3_4M.md", "c4": "step_kernels_c4_9M.md"}
+PROFILE = {"c2": "step_kernels_c2_1M.md", "c3": "step_kernels_c3_4M.md", "c4": "step_kernels_c4_9M.md",
+           "c5": "step_kernels_c5_serving.md"}
 
 
 def profiled_traffic(cfg, kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
     (FETCH_SIZE and WRITE_SIZE scaled by a plain copy of known size), with
-    that summary's average duration; (None, None) without a summary."""
+    that summary's average duration; (None, None, None) without a summary."""
     name = PROFILE.get(cfg)
     path = os.path.join(ROOT, "profiles", name) if name else None
     if not path or not os.path.exists(path):
@@ -127,7 +145,9 @@ def stage_graph_bytes(e, steps, ps):
     return n * (140 + 16 * P) + 12 * lay["total_buckets"] + 336 * lay["total_streams"]
 
 
-def roofline(e, reps, cfg, nprof=5):
+def roofline(e, cfg, nprof=5):
+    """Per-stage HIP-event times of `nprof` eager steps (on the engine's
+    stream) and the dominant single kernel's roofline."""
     prof = e.profile(nprof)
     lay = e.layout()
     n = int(e.counts().sum())
@@ -138,7 +158,6 @@ def roofline(e, reps, cfg, nprof=5):
     for k, us in prof.items():
         b = stage_bytes(k, n, S, lay["total_buckets"], passes)
         rows[k] = dict(us=us, bytes=b, gbs=(b / (us * 1e-6) / 1e9) if us > 0 else 0.0)
-    # the dominant kernel: the longest stage that is a single kernel
     dom = max((k for k in rows if k in SINGLE_KERNEL), key=lambda k: rows[k]["us"])
     step_us = sum(r["us"] for r in rows.values())
     step_bytes = sum(r["bytes"] for r in rows.values())
@@ -150,35 +169,150 @@ def roofline(e, reps, cfg, nprof=5):
           "profile": prof_file, "profile_avg_us": prof_us}
     step = {"us": round(step_us, 2), "bytes": step_bytes,
             "gbs": round(step_bytes / (step_us * 1e-6) / 1e9, 1) if step_us > 0 else 0.0,
-            "stages_us": {k: round(v["us"], 2) for k, v in rows.items()}}
+            "stages_us": {k: round(v["us"], 2) for k, v in rows.items()},
+            "stages_frac": {k: round(v["gbs"] / HBM_PEAK_GBS, 4) for k, v in rows.items() if v["bytes"]}}
     return rl, step
 
 
-def cpu_baseline(reps, budget_s=20.0, threads=0):
-    """The oracle (C++ -O3) stepping the first steps of the workload for about
-    budget_s seconds, on `threads` OpenMP threads (0: all host cores)."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def host_cores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def load_oracle():
+    """The oracle built with -march=native on this host (a few seconds of
+    g++), or the prebuilt x86-64-v3 one if that build fails."""
+    native_dir = os.path.join(ROOT, "oracle", "_build_native")
+    flags = "-O3 -march=native"
+    try:
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "-s", "native", f"NATIVE_DIR={native_dir}"],
+                       check=True, capture_output=True, timeout=120)
+        os.environ["SMD_ORACLE_LIB"] = os.path.join(native_dir, "libsmdoracle.so")
+    except (subprocess.SubprocessError, OSError) as ex:
+        log(f"native oracle build failed ({ex}); using the prebuilt -march=x86-64-v3 oracle")
+        flags = "-O3 -march=x86-64-v3"
     import oracle
 
+    oracle.lib()
+    return oracle, flags
+
+
+def cpu_baseline(reps, steps, budget_s, threads):
+    """The oracle on a bounded sample of the workload: one replica stepped on
+    `threads` OpenMP threads until budget_s seconds or `steps` steps; or, for
+    an ensemble, its replicas in order for `steps` steps each, `threads` of
+    them at a time on one thread each, as many as the budget allows (oracle
+    set-up outside the timed region).  Returns the report dict."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    oracle, flags = load_oracle()
     L = oracle.lib()
-    all_threads = L.orc_num_threads()
-    L.orc_set_threads(threads or all_threads)
-    spec, pos, types = reps[0]
-    sim = oracle.sim_for(smd, spec, pos, types, cap=4 * len(types) + 1024)
     g = smd.gauss_table()
-    done_ps, t_used, k = 0, 0.0, 0
-    while t_used < budget_s and k < 1000:
-        n = sim.n
+    model = cpu_model()
+    if len(reps) == 1:
+        L.orc_set_threads(threads)
+        spec, pos, types = reps[0]
+        sim = oracle.sim_for(smd, spec, pos, types, cap=4 * len(types) + 1024)
+        done_ps, k = 0, 0
         t0 = time.perf_counter()
-        sim.step(g, 1)
-        t_used += time.perf_counter() - t0
-        done_ps += n
-        k += 1
-    cores = L.orc_num_threads()
-    L.orc_set_threads(all_threads)
-    return {"value": done_ps / t_used, "unit": "particle-steps/s", "cores": cores,
-            "kind": "port",
-            "sample": f"CPU oracle (C++ -O3, OpenMP producer/pair search, {cores} threads), first {k} steps of "
-                      f"the same workload ({len(types)} particles), {t_used:.1f} s"}
+        while time.perf_counter() - t0 < budget_s and k < steps:
+            done_ps += sim.n
+            sim.step(g, 1)
+            k += 1
+        wall = time.perf_counter() - t0
+        sample = f"first {k} steps of the workload ({len(types)} particles) on {threads} OpenMP threads"
+    else:
+        def make(rep):
+            spec, pos, types = rep
+            return oracle.sim_for(smd, spec, pos, types, cap=4 * len(types) + 1024)
+
+        def run(sim):
+            L.orc_set_threads(1)
+            ps = 0
+            for _ in range(steps):
+                ps += sim.n
+                sim.step(g, 1)
+            return ps
+
+        # estimate the replicas the budget holds from the first `threads` of them
+        sims = [make(r) for r in reps[:threads]]
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(max_workers=threads) as ex:
+            done_ps = sum(ex.map(run, sims))
+        first = time.perf_counter() - t0
+        n_first = sum(len(r[2]) for r in reps[:threads])
+        used = threads
+        if first < budget_s:
+            per_particle = first / max(n_first, 1) / threads  # wall seconds per particle, `threads` in flight
+            more, acc = [], 0.0
+            for r in reps[threads:]:
+                acc += len(r[2]) * per_particle
+                if first + acc > budget_s:
+                    break
+                more.append(r)
+            sims = [make(r) for r in more]
+            t1 = time.perf_counter()
+            with ThreadPoolExecutor(max_workers=threads) as ex:
+                done_ps += sum(ex.map(run, sims))
+            first += time.perf_counter() - t1
+            used += len(more)
+        wall = first
+        sample = (f"replicas 0..{used - 1} of the {len(reps)} ({sum(len(r[2]) for r in reps[:used])} particles), "
+                  f"{steps} steps each, {threads} at a time on one thread each")
+    return {"value": done_ps / wall, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"CPU oracle (C++ {flags}, {model}, {host_cores()} host cores): {sample}, {wall:.1f} s",
+            "cpu_model": model}
+
+
+def run_steps(cfg, args, reps=None):
+    """Engine over the workload, W warm-up steps, K timed steps; returns
+    (engine, particle-steps, seconds, description)."""
+    r, default_steps, desc = build(cfg)
+    reps = reps or r
+    steps = args.steps if args.steps is not None else default_steps
+    e = make_engine(reps, args.radix_bits)
+    if args.no_graphs:
+        e.set_graphs(False)
+    if args.sort != 1:
+        e.set_sort(args.sort)
+    e.step(args.warmup)
+    ps, secs = timed(e, steps)
+    return e, reps, steps, ps, secs, desc
+
+
+def plain_forms(reps, steps, warmup, radix_bits):
+    """The batched workload run one simulation per launch sequence: each
+    replica in its own engine, W warm-up steps, then K eager steps and K
+    graph-replayed steps timed separately.  Returns the totals."""
+    eager_ps = eager_s = graph_ps = graph_s = 0.0
+    for k, rep in enumerate(reps):
+        p = make_engine([rep], radix_bits)
+        p.set_graphs(False)
+        p.step(warmup)
+        a, b = timed(p, steps)
+        eager_ps += a
+        eager_s += b
+        p.set_graphs(True)
+        p.step(2)  # capture outside the timed region, as in the batched run's warm-up
+        a, b = timed(p, steps)
+        graph_ps += a
+        graph_s += b
+        p.close()
+        if k % 128 == 127:
+            log(f"plain replicas done: {k + 1}/{len(reps)}")
+    return eager_ps, eager_s, graph_ps, graph_s
 
 
 def serve(args):
@@ -190,7 +324,20 @@ def serve(args):
     srv.run([serving.make_job(-1, 0.0, 4096, 20, 3, 1)], realtime=False)
     results, st = srv.run(jobs, realtime=True)
     srv.close()
-    n_ps = st["particle_steps"]
+    return jobs, st
+
+
+def serving_report(jobs, st):
+    return {"p50_latency_s": round(st["p50_s"], 4), "p99_latency_s": round(st["p99_s"], 4),
+            "particle_steps_per_s": st["particle_steps_per_s"], "wall_s": round(st["wall_s"], 2),
+            "jobs": st["jobs"], "engine_steps": st["engine_steps"], "last_arrival_s": round(jobs[-1].arrival, 2),
+            "workload": f"c5: {len(jobs)} jobs, Poisson 20 jobs/s, n log-uniform 1k-256k, steps log-uniform "
+                        "100-10k, 2-4 species, random A+B<->C / decay tables, one stream per 2 particles; "
+                        "32 slots of 2^18 buckets"}
+
+
+def c5_line(args):
+    jobs, st = serve(args)
     return {
         "metric": METRIC,
         "value": st["particle_steps_per_s"],
@@ -204,53 +351,15 @@ def serve(args):
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"c5: {len(jobs)} jobs, Poisson 20 jobs/s, n log-uniform 1k-256k, steps log-uniform "
-                               "100-10k, 2-4 species, random A+B<->C / decay tables, one stream per 2 particles; "
-                               "32 slots of 2^18 buckets",
-                   "jobs": len(jobs), "particle_steps": n_ps},
-        "serving": {"p50_latency_s": st["p50_s"], "p99_latency_s": st["p99_s"], "wall_s": st["wall_s"],
-                    "jobs": st["jobs"], "last_arrival_s": jobs[-1].arrival},
+        "config": {"workload": serving_report(jobs, st)["workload"], "jobs": len(jobs),
+                   "particle_steps": st["particle_steps"]},
+        "serving": serving_report(jobs, st),
         "roofline": None,
     }
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="c2")
-    ap.add_argument("--radix-bits", type=int, default=0, help="sort digit width (4, 8, 11; 0 = automatic)")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-plain", action="store_true", help="c4: skip the one-replica-per-launch comparison")
-    ap.add_argument("--no-graphs", action="store_true", help="one launch sequence per step instead of graphs")
-    ap.add_argument("--sort", type=int, default=1, help="1: count/scan/scatter (default), 0: single-sweep radix sort")
-    ap.add_argument("--cpu-budget", type=float, default=20.0)
-    ap.add_argument("--jobs", type=int, default=200, help="c5: number of jobs")
-    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
-                    help="also time a plain copy kernel over BYTES (reference for hardware byte counters)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); replicas and slabs would shard "
-              "across GPUs (a replica set or a spatial slab per GPU), but this build runs on a single MI355X.", file=sys.stderr)
-        sys.exit(2)
-    if args.config == "c5":
-        print(json.dumps(serve(args)), flush=True)
-        return
-    probe = None
-    if args.copy_probe:
-        us = smd.copy_probe(args.copy_probe, reps=10)
-        probe = {"bytes": args.copy_probe, "avg_us": us, "gbs": 2 * args.copy_probe / (us * 1e-6) / 1e9}
-    reps, steps, desc = build(args.config)
-    steps = args.steps if args.steps is not None else steps
-    e = make_engine(reps, args.radix_bits)
-    if args.no_graphs:
-        e.set_graphs(False)
-    if args.sort != 1:
-        e.set_sort(args.sort)
-    e.step(args.warmup)
-    ps, secs = timed(e, steps)
-    rl, step = roofline(e, reps, args.config)
+def base_line(cfg, args, e, steps, ps, secs, desc):
+    rl, step = roofline(e, cfg)
     out = {
         "metric": METRIC,
         "value": ps / secs,
@@ -271,27 +380,75 @@ def main():
     b_step = stage_graph_bytes(e, steps, ps)
     gbs = b_step / (secs / max(steps, 1)) / 1e9
     out["step_stage_graph"] = {"bytes": int(b_step), "gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="all")
+    ap.add_argument("--radix-bits", type=int, default=0, help="sort digit width (4, 8, 11; 0 = automatic)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="c4: skip the one-replica-per-launch comparison")
+    ap.add_argument("--no-serving", action="store_true", help="all: skip the c5 serving run")
+    ap.add_argument("--no-c3", action="store_true", help="all: skip the c3 step")
+    ap.add_argument("--no-graphs", action="store_true", help="one launch sequence per step instead of graphs")
+    ap.add_argument("--sort", type=int, default=1, help="1: count/scan/scatter (default), 0: single-sweep radix sort")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--jobs", type=int, default=200, help="c5: number of jobs")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="also time a plain copy kernel over BYTES (reference for hardware byte counters)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); replicas and slabs would shard "
+              "across GPUs (a replica set or a spatial slab per GPU), but this build runs on a single MI355X.",
+              file=sys.stderr)
+        sys.exit(2)
+    if args.config == "c5":
+        print(json.dumps(c5_line(args)), flush=True)
+        return
+    probe = None
+    if args.copy_probe:
+        us = smd.copy_probe(args.copy_probe, reps=10)
+        probe = {"bytes": args.copy_probe, "avg_us": us, "gbs": 2 * args.copy_probe / (us * 1e-6) / 1e9}
+    cfg = "c4" if args.config == "all" else args.config
+    if args.config == "all" and args.steps is None:
+        args.steps = 20
+    log(f"{cfg}: building the workload")
+    e, reps, steps, ps, secs, desc = run_steps(cfg, args)
+    log(f"{cfg}: {steps} steps in {secs * 1e3:.2f} ms")
+    out = base_line(cfg, args, e, steps, ps, secs, desc)
+    e.close()
     if probe:
         out["copy_probe"] = probe
-    if args.config == "c4" and not args.no_plain:
-        # the same 512-replica workload, one simulation per launch sequence
-        pps, psecs = 0, 0.0
-        for k, rep in enumerate(reps):
-            p = make_engine([rep], args.radix_bits)
-            p.set_graphs(False)
-            p.step(args.warmup)
-            a, b = timed(p, steps)
-            pps += a
-            psecs += b
-            p.close()
-            if k % 64 == 63:
-                print(f"[bench] plain replicas done: {k + 1}/{len(reps)}", file=sys.stderr, flush=True)
-        plain = pps / psecs
-        out["batched_vs_plain"] = {"batched": out["value"], "plain": plain, "ratio": out["value"] / plain,
-                                   "plain_seconds": psecs, "batched_seconds": secs}
+    if cfg == "c4" and not args.no_plain:
+        log("c4: plain forms (each replica alone)")
+        eps, es, gps, gs = plain_forms(reps, steps, args.warmup, args.radix_bits)
+        out["batched_vs_plain"] = {
+            "batched": out["value"], "plain_eager": eps / es, "plain_graph": gps / gs,
+            "ratio_vs_eager": out["value"] / (eps / es), "ratio_vs_graph": out["value"] / (gps / gs),
+            "batched_seconds": secs, "plain_eager_seconds": es, "plain_graph_seconds": gs,
+            "note": "plain: each replica in its own engine, one launch sequence per step (eager) or two-step "
+                    "hipGraphs (graph), K steps each after the same warm-up"}
+    if args.config == "all" and not args.no_c3:
+        log("c3: 4M-particle clustered step")
+        e3, _, s3, ps3, secs3, d3 = run_steps("c3", args)
+        l3 = base_line("c3", args, e3, s3, ps3, secs3, d3)
+        e3.close()
+        out["c3_4M"] = {"workload": d3["workload"], "steps": s3, "ms_per_step": l3["ms_per_step"],
+                        "particle_steps_per_s": l3["value"], "stage_graph": l3["step_stage_graph"],
+                        "roofline": l3["roofline"], "stages_us": l3["step_roofline"]["stages_us"]}
+    if args.config == "all" and not args.no_serving:
+        log("c5: serving stream")
+        jobs, st = serve(args)
+        out["serving"] = serving_report(jobs, st)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(reps, args.cpu_budget)
-        out["cpu_baseline_single_thread"] = cpu_baseline(reps, args.cpu_budget / 2, threads=1)
+        log("CPU baseline")
+        threads = host_cores()
+        out["cpu_baseline"] = cpu_baseline(reps, steps, args.cpu_budget, threads)
+        out["cpu_baseline_single_thread"] = cpu_baseline(reps, steps, args.cpu_budget / 2, 1)
     print(json.dumps(out), flush=True)
 
 

@@ -11,7 +11,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "_build", "libsmdoracle.so")
+# SMD_ORACLE_LIB selects another build of the same source (the benchmark's
+# CPU baseline uses one built with -march=native on the host that runs it)
+LIB = os.environ.get("SMD_ORACLE_LIB") or os.path.join(HERE, "_build", "libsmdoracle.so")
 REF_LIB = os.path.join(HERE, "_ref", "libsmdref.so")
 
 MAXT, MAXR, MAXS = 16, 32, 4
