@@ -147,6 +147,7 @@ void Engine::free_particle_buffers() {
   dfree(dTileBirth);
   dfree(dTileAlivePre);
   dfree(dTileBirthPre);
+  dfree(dBlockSum);
 }
 
 int Engine::add_replica(const smd_sim_spec& spec, const float* pos4, const int* types, uint32_t n) {
@@ -194,6 +195,8 @@ smd_status Engine::alloc_particle_buffers(uint32_t c) {
   HIPCHK(dalloc(&dTileBirth, nTilesCap));
   HIPCHK(dalloc(&dTileAlivePre, nTilesCap));
   HIPCHK(dalloc(&dTileBirthPre, nTilesCap));
+  blockStride = finalize_blocks(cap);
+  HIPCHK(dalloc(&dBlockSum, 2 * (size_t)blockStride));
   HIPCHK(hipMemsetAsync(dTileAlive, 0, nTilesCap * sizeof(uint32_t), stream));
   HIPCHK(hipMemsetAsync(dTileBirth, 0, nTilesCap * sizeof(uint32_t), stream));
   return SMD_OK;
@@ -570,7 +573,7 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
   pa.tileAlive = dTileAlive;
   pa.tileBirth = dTileBirth;
   pa.retire = dRetire;
-  HIPCHK(launch_producer(pa, stream));
+  HIPCHK(launch_producer(pa, producerBlock, stream));
   if (anyZero) {
     ZeroArgs za;
     za.tables = dTables;
@@ -602,7 +605,7 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
     ca.first = dFirst;
     ca.dLive = dLive;
     ca.cand = dCand;
-    HIPCHK(launch_candidates(ca, maxN, stream));
+    HIPCHK(launch_candidates(ca, maxN, pairBlock, stream));
     mark(4);
     ResolveArgs ra;
     ra.tables = dTables;
@@ -648,7 +651,9 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
   fa.zeroBase = dZeroBase;
   fa.injBase = dInjBase;
   fa.retire = dRetire;
-  HIPCHK(launch_finalize(fa, stream));
+  fa.blockSum = dBlockSum;
+  fa.blockStride = blockStride;
+  HIPCHK(launch_finalize(fa, maxN, stream));
   mark(6);
   ScatterArgs sa;
   sa.R = (int)R;
@@ -666,6 +671,8 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
   sa.birthType = dBirthType;
   sa.ptOut = dPT[cur ^ 1];
   sa.wOut = dW[cur ^ 1];
+  sa.blockSum = dBlockSum;
+  sa.blockStride = blockStride;
   HIPCHK(launch_scatter(sa, maxN, stream));
   if (anyZero)
     HIPCHK(launch_zero_place((int)R, dTables, dZPos, dZType, dZCount, dNewOff, dZeroBase, dPT[cur ^ 1],
@@ -746,6 +753,27 @@ smd_status Engine::check_status() {
   if (v & kStatusSortSpin) {
     set_error("sort look-back wait exceeded its bound");
     return SMD_ERR_HIP;
+  }
+  return SMD_OK;
+}
+
+smd_status Engine::set_knob(int knob, int value) {
+  int* slot = nullptr;
+  bool ok = false;
+  if (knob == SMD_KNOB_PRODUCER_BLOCK) {
+    slot = &producerBlock;
+    ok = value == 128 || value == 256 || value == 512;
+  } else if (knob == SMD_KNOB_PAIR_BLOCK) {
+    slot = &pairBlock;
+    ok = value == 64 || value == 128 || value == 256;
+  }
+  if (!slot || !ok) {
+    set_error("unknown knob or value out of range");
+    return SMD_ERR_INVALID;
+  }
+  if (*slot != value) {
+    drop_graph();  // captured launches keep the old shape
+    *slot = value;
   }
   return SMD_OK;
 }
@@ -1045,7 +1073,7 @@ smd_status Engine::debug_stage(int stage, float* pos4, int* types, uint32_t* key
   pa.tileAlive = dTileAlive;
   pa.tileBirth = dTileBirth;
   pa.retire = nullptr;
-  HIPCHK(launch_producer(pa, stream));
+  HIPCHK(launch_producer(pa, producerBlock, stream));
   uint32_t live = 0;
   if (stage >= 2) {
     sb.gatherSrc = dPT[cur];
@@ -1065,7 +1093,7 @@ smd_status Engine::debug_stage(int stage, float* pos4, int* types, uint32_t* key
     ca.first = dFirst;
     ca.dLive = dLive;
     ca.cand = dCand;
-    HIPCHK(launch_candidates(ca, maxN, stream));
+    HIPCHK(launch_candidates(ca, maxN, pairBlock, stream));
   }
   if (stage >= 4) {
     ResolveArgs ra;

@@ -85,13 +85,14 @@ struct ProducerLds {
 // The replica tables arrive as a separate read-only, non-aliased argument:
 // their wave-uniform fields then load through the scalar cache, whose waits
 // do not hold up the particle records in flight.
-__global__ __launch_bounds__(kBS) void k_producer(ProducerArgs a, const ReplicaTables* __restrict__ tables) {
-  __shared__ uint32_t lds_mt[kMtNN * kBS];
+template <int BS>
+__global__ __launch_bounds__(BS) void k_producer(ProducerArgs a, const ReplicaTables* __restrict__ tables) {
+  __shared__ uint32_t lds_mt[kMtNN * BS];
   __shared__ float lds_gauss[kGaussDim];
-  __shared__ ProducerLds lds_tab[kBS / 64];
-  for (int k = threadIdx.x; k < kGaussDim; k += kBS) lds_gauss[k] = a.gauss[k];
+  __shared__ ProducerLds lds_tab[BS / 64];
+  for (int k = threadIdx.x; k < kGaussDim; k += BS) lds_gauss[k] = a.gauss[k];
 
-  const uint32_t g = blockIdx.x * kBS + threadIdx.x;
+  const uint32_t g = blockIdx.x * BS + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63u;
   const int r = (g < a.Stot) ? a.streamReplica[g] : -1;
   const int rw = __builtin_amdgcn_readfirstlane(r);  // a wave never spans two replicas
@@ -133,7 +134,7 @@ __global__ __launch_bounds__(kBS) void k_producer(ProducerArgs a, const ReplicaT
     };
     float4 p0 = rec(0), p1 = rec(1);
     Mt m;
-    if (laneOK && !kill) mt_load<kBS>(m, &lds_mt[threadIdx.x], a.mtState, a.mtPar, a.Stot, g);
+    if (laneOK && !kill) mt_load<BS>(m, &lds_mt[threadIdx.x], a.mtState, a.mtPar, a.Stot, g);
     // the stream parameters are complete before the loop, so no wait inside
     // it covers them (such a wait would also cover the records in flight)
     asm volatile("" ::"v"(m.a), "v"(m.mb), "v"(m.mc), "v"(m.mti1));
@@ -153,7 +154,7 @@ __global__ __launch_bounds__(kBS) void k_producer(ProducerArgs a, const ReplicaT
         const float4 p = pCur;
         int t = __float_as_int(pCur.w);
         uint32_t u0, u1, u2;
-        mt_draw3<kBS>(m, u0, u1, u2);
+        mt_draw3<BS>(m, u0, u1, u2);
         const float rt = L.rate[t > 0 ? t : 0];
         float np[3];
         np[0] = fmaf(rt, lds_gauss[u0 & (kGaussDim - 1)], p.x);
@@ -170,7 +171,7 @@ __global__ __launch_bounds__(kBS) void k_producer(ProducerArgs a, const ReplicaT
           const int base = L.uniStart[t];
           for (int j = 0; j < cnt; ++j) {
             const int rx = L.uniList[base + j];
-            if (rand_real(mt_draw<kBS>(m)) < L.prob[rx]) {
+            if (rand_real(mt_draw<BS>(m)) < L.prob[rx]) {
               if (L.prodX[rx] <= 0) {
                 t = 0;
               } else {
@@ -179,9 +180,9 @@ __global__ __launch_bounds__(kBS) void k_producer(ProducerArgs a, const ReplicaT
                 if (t2 > 0) {
                   bt = t2;
                   if (T.unbindOffsets) {
-                    const float th = smd_theta(rand_real(mt_draw<kBS>(m)));
-                    const float ph = rand_real(mt_draw<kBS>(m)) * 6.28318548f;
-                    const float ch = rand_real(mt_draw<kBS>(m)) * 6.28318548f;
+                    const float th = smd_theta(rand_real(mt_draw<BS>(m)));
+                    const float ph = rand_real(mt_draw<BS>(m)) * 6.28318548f;
+                    const float ch = rand_real(mt_draw<BS>(m)) * 6.28318548f;
                     float v1[3], v2[3];
                     rotate_offsets(th, ph, ch, T.prodPos[rx], v1, v2);
                     bp[0] = np[0] + v2[0];
@@ -251,7 +252,7 @@ __global__ __launch_bounds__(kBS) void k_producer(ProducerArgs a, const ReplicaT
     if (it < iters) db += step_one(p0, it);
     nDeaths += (uint32_t)db;
     nBirths += (uint32_t)(db >> 32);
-    if (laneOK && !kill) mt_store<kBS>(m, a.mtState, a.Stot, g);
+    if (laneOK && !kill) mt_store<BS>(m, a.mtState, a.Stot, g);
   }
   nDeaths = wave_sum(nDeaths);
   nBirths = wave_sum(nBirths);
@@ -964,21 +965,38 @@ __device__ __forceinline__ void wave_search(const ReplicaTables& T, const CandAr
                     !(lo[0] && gx == 0u) && !(hi[0] && gx == mx);
   const uint32_t kLo = rows ? key - (uint32_t)lo[0] : key;
   const uint32_t kHi = rows ? key + (uint32_t)hi[0] : key;
-  // the key range inside the wave
-  for (int o = 1; o < 64; ++o) {
-    const int up = (int)lane + o, dn = (int)lane - o;
-    const uint32_t kUp = __shfl(key, up & 63, 64);
-    const uint32_t kDn = __shfl(key, dn & 63, 64);
-    // only lanes that have reactive partners keep the loop going
-    const bool hitUp = mask && up < 64 && kUp <= kHi;
-    const bool hitDn = mask && dn >= 0 && kDn >= kLo;
-    if (!__any(hitUp || hitDn)) break;  // keys ascend: no lane has a partner further out
-    const float4 qUp = make_float4(__shfl(p.x, up & 63, 64), __shfl(p.y, up & 63, 64), __shfl(p.z, up & 63, 64),
-                                   __shfl(p.w, up & 63, 64));
-    const float4 qDn = make_float4(__shfl(p.x, dn & 63, 64), __shfl(p.y, dn & 63, 64), __shfl(p.z, dn & 63, 64),
-                                   __shfl(p.w, dn & 63, 64));
-    if (hitUp) consider(T, qUp, wbase + (uint32_t)up, j, p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
-    if (hitDn) consider(T, qDn, wbase + (uint32_t)dn, j, p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
+  // The key range inside the wave.  Keys ascend through the lanes, so each
+  // key is a run of consecutive lanes and [kLo, kHi] (at most three
+  // consecutive keys) is one stretch of lanes [s, e].  Runs start where the
+  // key changes; the stretch is the lane's own run, extended by the run
+  // before (after) it when that run holds key kLo (kHi).
+  const uint32_t keyPrev = __shfl(key, (int)((lane + 63u) & 63u), 64);
+  const uint64_t starts = __ballot(lane == 0u || key != keyPrev);
+  const uint64_t upTo = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);  // lanes <= lane
+  const int s0 = 63 - __clzll(starts & upTo);  // bit 0 is always set
+  const uint64_t after0 = starts & ~upTo;
+  const int e0 = after0 ? __ffsll((unsigned long long)after0) - 2 : 63;
+  const uint32_t kBefore = __shfl(key, s0 > 0 ? s0 - 1 : 0, 64);
+  const uint32_t kAfter = __shfl(key, e0 < 63 ? e0 + 1 : 63, 64);
+  int s = s0, e = e0;
+  if (kLo < key && s0 > 0 && kBefore == kLo) s = 63 - __clzll(starts & ((1ull << s0) - 1ull));
+  if (kHi > key && e0 < 63 && kAfter == kHi) {
+    const uint64_t after1 = e0 + 1 == 63 ? 0ull : (starts & ~((2ull << (e0 + 1)) - 1ull));
+    e = after1 ? __ffsll((unsigned long long)after1) - 2 : 63;
+  }
+  // Every other lane of the stretch once, by rotation: at step o the lane
+  // reads s + ((lane - s + o) mod len); one exchange per step.  Only lanes
+  // with reactive partners keep the loop going.
+  const int len = e - s + 1;
+  const int myLen = mask ? len : 1;
+  for (int o = 1; __any(o < myLen); ++o) {
+    const bool take = o < myLen;
+    int t = (int)lane - s + o;
+    if (t >= len) t -= len;
+    const int src = take ? s + t : (int)lane;
+    const float4 q = make_float4(__shfl(p.x, src, 64), __shfl(p.y, src, 64), __shfl(p.z, src, 64),
+                                 __shfl(p.w, src, 64));
+    if (take) consider(T, q, wbase + (uint32_t)src, j, p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
   }
   // the wave's first and last keys, read while every lane is still active
   const uint32_t key0 = __shfl(key, 0, 64), key63 = __shfl(key, 63, 64);
@@ -1043,13 +1061,14 @@ __device__ __forceinline__ void wave_search(const ReplicaTables& T, const CandAr
 // The bucket offsets and replica tables arrive as read-only non-aliased
 // arguments, so the wave's replica (searched once, for its first key) and the
 // table fields load through the scalar cache.
-__global__ __launch_bounds__(256) void k_candidates(CandArgs a, const uint32_t* __restrict__ bucketStart,
-                                                    const ReplicaTables* __restrict__ tables) {
+template <int BS>
+__global__ __launch_bounds__(BS) void k_candidates(CandArgs a, const uint32_t* __restrict__ bucketStart,
+                                                   const ReplicaTables* __restrict__ tables) {
   const uint32_t nLive = *a.dLive;
-  const uint32_t nBlocks = (nLive + 255u) / 256u;
+  const uint32_t nBlocks = (nLive + BS - 1u) / BS;
   if (blockIdx.x >= nBlocks) return;
   // neighbouring blocks on one XCD: the cells they both search share its L2
-  const uint32_t j = xcd_tile(blockIdx.x, nBlocks) * 256 + threadIdx.x;
+  const uint32_t j = xcd_tile(blockIdx.x, nBlocks) * BS + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wbase = j - lane;
   if (wbase >= nLive) return;  // the whole wave is past the live particles
@@ -1227,102 +1246,106 @@ __device__ __forceinline__ void block_exscan1024_k(const uint32_t (&v)[K], uint3
   __syncthreads();
 }
 
+// Compaction tiles per finalize workgroup (1024 threads x 4 consecutive tiles).
+constexpr uint32_t kFinTPT = 4;
+constexpr uint32_t kFinBlockTiles = 1024 * kFinTPT;
+
+// Per-step bookkeeping after the pair step.  Workgroup g scans the compaction
+// tiles [g * 4096, (g + 1) * 4096): exclusive prefixes of both counters inside
+// the block, the counters reset, the block's totals published for the
+// scatter (which adds the totals of the earlier blocks).  Workgroup 0 also
+// lays out every replica's new segment and publishes the step's counts.
 __global__ __launch_bounds__(1024) void k_finalize(FinalizeArgs a) {
   __shared__ uint32_t wsum[3][16];
   const uint32_t n = *a.dN;  // read before any store (the arguments may alias)
-  // the first round of compaction-tile counters is requested right away
-  constexpr uint32_t kTPT = 4;  // consecutive tiles per thread
   const uint32_t nTiles = (n + kCompactTile - 1) / kCompactTile;
-  uint32_t va[kTPT], vb[kTPT];
+  const uint32_t t0 = blockIdx.x * kFinBlockTiles;
+  if (blockIdx.x > 0 && t0 >= nTiles) return;
+  // this block's compaction-tile counters are requested right away
+  uint32_t va[kFinTPT], vb[kFinTPT];
+  const uint32_t e0 = t0 + threadIdx.x * kFinTPT;
 #pragma unroll
-  for (uint32_t k = 0; k < kTPT; ++k) {
-    const uint32_t e = threadIdx.x * kTPT + k;
+  for (uint32_t k = 0; k < kFinTPT; ++k) {
+    const uint32_t e = e0 + k;
     va[k] = e < nTiles ? a.tileAlive[e] : 0u;
     vb[k] = e < nTiles ? a.tileBirth[e] : 0u;
   }
-  // replicas
-  uint32_t cN = 0, cA = 0, cB = 0;
-  for (uint32_t b = 0; b < (uint32_t)a.R; b += 1024) {
-    const uint32_t r = b + threadIdx.x;
-    uint32_t alive = 0, births = 0, zc = 0, inj = 0;
-    if (r < (uint32_t)a.R) {
-      alive = a.repCount[r] - a.repDeaths[r];
-      births = a.repBirths[r];
-      if (a.zCount) zc = a.zCount[r];
-      if (a.injCount) inj = a.injCount[r];
-    }
-    // new segment of replica r: [survivors, births, created, admitted]
-    const uint32_t v[3] = {alive + births + zc + inj, alive, births};
-    uint32_t ex[3], tot[3];
-    block_exscan1024_k<3>(v, ex, tot, wsum);
-    if (r < (uint32_t)a.R) {
-      a.newOff[r] = cN + ex[0];
-      a.newCount[r] = v[0];
-      a.aliveBase[r] = cA + ex[1];
-      a.birthBase[r] = cB + ex[2];
-      a.repAlive[r] = alive;
-      a.repDeaths[r] = 0;
-      a.repBirths[r] = 0;
-      if (a.zCount) a.zeroBase[r] = alive + births;
-      if (a.injCount) {
-        a.injBase[r] = alive + births + zc;
-        a.injCount[r] = 0;
+  if (blockIdx.x == 0) {
+    // replicas
+    uint32_t cN = 0, cA = 0, cB = 0;
+    for (uint32_t b = 0; b < (uint32_t)a.R; b += 1024) {
+      const uint32_t r = b + threadIdx.x;
+      uint32_t alive = 0, births = 0, zc = 0, inj = 0;
+      if (r < (uint32_t)a.R) {
+        alive = a.repCount[r] - a.repDeaths[r];
+        births = a.repBirths[r];
+        if (a.zCount) zc = a.zCount[r];
+        if (a.injCount) inj = a.injCount[r];
       }
-      if (a.retire) a.retire[r] = 0;
+      // new segment of replica r: [survivors, births, created, admitted]
+      const uint32_t v[3] = {alive + births + zc + inj, alive, births};
+      uint32_t ex[3], tot[3];
+      block_exscan1024_k<3>(v, ex, tot, wsum);
+      if (r < (uint32_t)a.R) {
+        a.newOff[r] = cN + ex[0];
+        a.newCount[r] = v[0];
+        a.aliveBase[r] = cA + ex[1];
+        a.birthBase[r] = cB + ex[2];
+        a.repAlive[r] = alive;
+        a.repDeaths[r] = 0;
+        a.repBirths[r] = 0;
+        if (a.zCount) a.zeroBase[r] = alive + births;
+        if (a.injCount) {
+          a.injBase[r] = alive + births + zc;
+          a.injCount[r] = 0;
+        }
+        if (a.retire) a.retire[r] = 0;
+      }
+      cN += tot[0];
+      cA += tot[1];
+      cB += tot[2];
     }
-    cN += tot[0];
-    cA += tot[1];
-    cB += tot[2];
+    if (threadIdx.x == 0) {
+      a.newOff[a.R] = cN;
+      *a.dNnext = cN;
+      *a.particleSteps += n;
+      // the host reads the particle count after each step from this ring,
+      // tagged with the step number (no copy or event in the stream)
+      const uint32_t st = *a.stepCount + 1u;
+      *a.stepCount = st;
+      __hip_atomic_store(&a.countRing[st & a.ringMask], ((unsigned long long)st << 32) | cN, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+  }
+  // this block's compaction tiles: exclusive prefixes inside the block,
+  // counters reset, block totals for the scatter
+  uint32_t sum[2] = {0u, 0u};
+#pragma unroll
+  for (uint32_t k = 0; k < kFinTPT; ++k) {
+    const uint32_t e = e0 + k;
+    sum[0] += va[k];
+    sum[1] += vb[k];
+    if (e < nTiles) {
+      a.tileAlive[e] = 0;
+      a.tileBirth[e] = 0;
+    }
+  }
+  uint32_t ex[2], tot[2];
+  block_exscan1024_k<2>(sum, ex, tot, wsum);
+  uint32_t ra = ex[0], rb = ex[1];
+#pragma unroll
+  for (uint32_t k = 0; k < kFinTPT; ++k) {
+    const uint32_t e = e0 + k;
+    if (e < nTiles) {
+      a.tileAlivePre[e] = ra;
+      a.tileBirthPre[e] = rb;
+    }
+    ra += va[k];
+    rb += vb[k];
   }
   if (threadIdx.x == 0) {
-    a.newOff[a.R] = cN;
-    *a.dNnext = cN;
-    *a.particleSteps += n;
-    // the host reads the particle count after each step from this ring,
-    // tagged with the step number (no copy or event in the stream)
-    const uint32_t st = *a.stepCount + 1u;
-    *a.stepCount = st;
-    __hip_atomic_store(&a.countRing[st & a.ringMask], ((unsigned long long)st << 32) | cN, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_SYSTEM);
-  }
-  // compaction tiles: exclusive prefixes of both counters, counters reset
-  uint32_t carA = 0, carB = 0;
-  for (uint32_t b = 0; b < nTiles; b += 1024 * kTPT) {
-    const uint32_t e0 = b + threadIdx.x * kTPT;
-    if (b > 0) {
-#pragma unroll
-      for (uint32_t k = 0; k < kTPT; ++k) {
-        const uint32_t e = e0 + k;
-        va[k] = e < nTiles ? a.tileAlive[e] : 0u;
-        vb[k] = e < nTiles ? a.tileBirth[e] : 0u;
-      }
-    }
-    uint32_t sum[2] = {0u, 0u};
-#pragma unroll
-    for (uint32_t k = 0; k < kTPT; ++k) {
-      const uint32_t e = e0 + k;
-      sum[0] += va[k];
-      sum[1] += vb[k];
-      if (e < nTiles) {
-        a.tileAlive[e] = 0;
-        a.tileBirth[e] = 0;
-      }
-    }
-    uint32_t ex[2], tot[2];
-    block_exscan1024_k<2>(sum, ex, tot, wsum);
-    uint32_t ra = carA + ex[0], rb = carB + ex[1];
-#pragma unroll
-    for (uint32_t k = 0; k < kTPT; ++k) {
-      const uint32_t e = e0 + k;
-      if (e < nTiles) {
-        a.tileAlivePre[e] = ra;
-        a.tileBirthPre[e] = rb;
-      }
-      ra += va[k];
-      rb += vb[k];
-    }
-    carA += tot[0];
-    carB += tot[1];
+    a.blockSum[blockIdx.x] = tot[0];
+    a.blockSum[a.blockStride + blockIdx.x] = tot[1];
   }
 }
 
@@ -1370,6 +1393,17 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a, const uint32_t* 
     cb += (uint32_t)__popcll(bm);
   }
   uint32_t pa = a.tileAlivePre[tile], pb = a.tileBirthPre[tile];
+  // totals of the finalize blocks before this tile's block
+  {
+    const uint32_t g = tile / kFinBlockTiles;
+    uint32_t sa = 0, sb = 0;
+    for (uint32_t q = lane; q < g; q += 64) {
+      sa += a.blockSum[q];
+      sb += a.blockSum[a.blockStride + q];
+    }
+    pa += wave_sum(sa);
+    pb += wave_sum(sb);
+  }
   if (lane == 0) {
     s_wa[w] = ca;
     s_wb[w] = cb;
@@ -1550,8 +1584,15 @@ hipError_t launch_mt_fill(uint32_t* st, const uint32_t* par, uint32_t S, uint32_
   return hipGetLastError();
 }
 
-hipError_t launch_producer(const ProducerArgs& a, hipStream_t stream) {
-  hipLaunchKernelGGL(k_producer, dim3(cdiv(a.Stot, kBS)), dim3(kBS), 0, stream, a, a.tables);
+hipError_t launch_producer(const ProducerArgs& a, int blockSize, hipStream_t stream) {
+  // the stream of global id g is g's in every shape: the results do not
+  // depend on the block size
+  if (blockSize == 512)
+    hipLaunchKernelGGL(k_producer<512>, dim3(cdiv(a.Stot, 512)), dim3(512), 0, stream, a, a.tables);
+  else if (blockSize == 128)
+    hipLaunchKernelGGL(k_producer<128>, dim3(cdiv(a.Stot, 128)), dim3(128), 0, stream, a, a.tables);
+  else
+    hipLaunchKernelGGL(k_producer<256>, dim3(cdiv(a.Stot, 256)), dim3(256), 0, stream, a, a.tables);
   return hipGetLastError();
 }
 
@@ -1645,14 +1686,23 @@ static hipError_t sort3k_pass(SortBuffers& b, const uint32_t* dN, uint32_t maxTi
   return hipGetLastError();
 }
 
+// 2048-key tiles up to 6M keys, 4096-key tiles beyond (measured: 1M-4M keys
+// sort 10-20% faster with the smaller tiles, 9M keys 3% slower).  The
+// expected count decides when the caller gives one (the bound maxN can be
+// several times the live count); the grid always covers maxN.
+static int sort3k_items(uint32_t maxN, uint32_t sizeHint) {
+  const uint32_t expect = sizeHint ? std::min(sizeHint, maxN) : maxN;
+  return expect <= 6000000u ? 8 : 16;
+}
+
+size_t sort3k_counts_words(uint32_t maxN, uint32_t sizeHint, int radixBits) {
+  const uint32_t maxTiles = cdiv(maxN, 256u * sort3k_items(maxN, sizeHint));
+  return ((size_t)1 << radixBits) * (maxTiles + 1) + ((size_t)1 << radixBits);
+}
+
 hipError_t launch_sort_3k(SortBuffers& b, const uint32_t* dN, uint32_t maxN, uint32_t keyBits, int radixBits,
                           bool identityVals, hipStream_t stream) {
-  // 2048-key tiles up to 6M keys, 4096-key tiles beyond (measured: 1M-4M
-  // keys sort 10-20% faster with the smaller tiles, 9M keys 3% slower).  The
-  // expected count decides when the caller gives one (the bound maxN can be
-  // several times the live count); the grid always covers maxN.
-  const uint32_t expect = b.sizeHint ? std::min(b.sizeHint, maxN) : maxN;
-  const int items = expect <= 6000000u ? 8 : 16;
+  const int items = sort3k_items(maxN, b.sizeHint);
   const uint32_t maxTiles = cdiv(maxN, 256u * items);
   if (maxTiles == 0) return hipSuccess;
   const uint32_t stride = maxTiles + 1;
@@ -1703,9 +1753,18 @@ hipError_t launch_gather(const uint32_t* keys, const uint32_t* vals, const float
   return hipGetLastError();
 }
 
-hipError_t launch_candidates(const CandArgs& a, uint32_t maxN, hipStream_t stream) {
+hipError_t launch_candidates(const CandArgs& a, uint32_t maxN, int blockSize, hipStream_t stream) {
   if (maxN == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_candidates, dim3(cdiv(maxN, 256)), dim3(256), 0, stream, a, a.bucketStart, a.tables);
+  // one result per sorted particle whatever the shape: waves are always 64
+  // consecutive particles
+  if (blockSize == 64)
+    hipLaunchKernelGGL(k_candidates<64>, dim3(cdiv(maxN, 64)), dim3(64), 0, stream, a, a.bucketStart, a.tables);
+  else if (blockSize == 128)
+    hipLaunchKernelGGL(k_candidates<128>, dim3(cdiv(maxN, 128)), dim3(128), 0, stream, a, a.bucketStart,
+                       a.tables);
+  else
+    hipLaunchKernelGGL(k_candidates<256>, dim3(cdiv(maxN, 256)), dim3(256), 0, stream, a, a.bucketStart,
+                       a.tables);
   return hipGetLastError();
 }
 
@@ -1714,8 +1773,14 @@ hipError_t launch_resolve(const ResolveArgs& a, hipStream_t stream) {
   return hipGetLastError();
 }
 
-hipError_t launch_finalize(const FinalizeArgs& a, hipStream_t stream) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, stream, a);
+uint32_t finalize_blocks(uint32_t maxN) {
+  return std::max<uint32_t>(1u, cdiv(cdiv(maxN, kCompactTile), kFinBlockTiles));
+}
+
+hipError_t launch_finalize(const FinalizeArgs& a, uint32_t maxN, hipStream_t stream) {
+  const uint32_t g = finalize_blocks(maxN);
+  if (g > a.blockStride) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_finalize, dim3(g), dim3(1024), 0, stream, a);
   return hipGetLastError();
 }
 

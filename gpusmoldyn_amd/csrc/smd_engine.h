@@ -78,6 +78,8 @@ class Engine {
   uint32_t viewCap = 0;
   SortBuffers sb = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 1, nullptr, nullptr, 0, 0};
   uint32_t *dTileAlive = nullptr, *dTileBirth = nullptr, *dTileAlivePre = nullptr, *dTileBirthPre = nullptr;
+  uint32_t* dBlockSum = nullptr;  // finalize block totals, [2][blockStride]
+  uint32_t blockStride = 0;
   uint32_t* dStatus = nullptr;  // sticky kernel error bits
   // order-0 creation: one stream per replica, a per-step creation buffer
   uint32_t *dZMt = nullptr, *dZPar = nullptr, *dZCount = nullptr, *dZeroBase = nullptr;
@@ -106,6 +108,10 @@ class Engine {
   uint32_t injCap = 0, injN = 0;
   std::vector<uint8_t> slotBusy;     // admitted and not yet retired
   std::vector<uint8_t> slotPending;  // admitted, particles not yet entered
+
+  // launch shapes (identical results for every setting; see set_knob)
+  int producerBlock = 256, pairBlock = 256;
+  smd_status set_knob(int knob, int value);
 
   // hipGraph replay of two-step pairs, one executable graph per ping-pong
   // parity (an eager step between pairs flips the parity)

@@ -97,6 +97,8 @@ struct FinalizeArgs {
   uint32_t* injCount;      // per replica: particles admitted this step (nullptr: none); reset
   uint32_t* injBase;       // per replica: alive + births + created, where admitted particles go
   uint32_t* retire;        // per replica retire flags (nullptr: none); reset
+  uint32_t* blockSum;      // [2][blockStride]: alive and birth totals of each 4096-tile block
+  uint32_t blockStride;    // at least finalize_blocks(maxN)
 };
 
 struct ZeroArgs {
@@ -138,16 +140,22 @@ struct ScatterArgs {
   const int* birthType;
   float4* ptOut;
   float* wOut;
+  const uint32_t* blockSum;  // the finalize blocks' totals (FinalizeArgs::blockSum)
+  uint32_t blockStride;
 };
 
 hipError_t launch_mt_init(uint32_t* st, const uint32_t* seeds, uint32_t S, hipStream_t stream);
 hipError_t launch_mt_fill(uint32_t* st, const uint32_t* par, uint32_t S, uint32_t nStreams, int count, int batch3,
                           uint32_t* out, hipStream_t stream);
-hipError_t launch_producer(const ProducerArgs& a, hipStream_t stream);
+// blockSize: 128, 256 or 512 threads (identical results)
+hipError_t launch_producer(const ProducerArgs& a, int blockSize, hipStream_t stream);
 int sort_passes(uint32_t keyBits, int radixBits);
 size_t sort_counts_words(uint32_t maxN, int radixBits);
 hipError_t launch_sort(SortBuffers& b, const uint32_t* dN, uint32_t maxN, uint32_t keyBits, int radixBits,
                        bool identityVals, hipStream_t stream);
+// Counter words the count / scan / scatter sort (SortBuffers::algo = 1)
+// uses for maxN keys: per-tile digit counts and the digit totals.
+size_t sort3k_counts_words(uint32_t maxN, uint32_t sizeHint, int radixBits);
 // first: first_words(nb) words (first[0..nb]).  slotBits > 0 (slot engines,
 // slots of 2^slotBits buckets): entries of slot s past offset
 // tables[s].numBuckets are not written.
@@ -157,9 +165,13 @@ hipError_t launch_first(const uint32_t* keys, const uint32_t* dN, uint32_t maxN,
                         hipStream_t stream);
 hipError_t launch_gather(const uint32_t* keys, const uint32_t* vals, const float4* src, const uint32_t* dN,
                          uint32_t maxN, uint32_t deadKey, float4* sortedPT, hipStream_t stream);
-hipError_t launch_candidates(const CandArgs& a, uint32_t maxN, hipStream_t stream);
+// blockSize: 64, 128 or 256 threads (identical results)
+hipError_t launch_candidates(const CandArgs& a, uint32_t maxN, int blockSize, hipStream_t stream);
 hipError_t launch_resolve(const ResolveArgs& a, hipStream_t stream);
-hipError_t launch_finalize(const FinalizeArgs& a, hipStream_t stream);
+// Finalize workgroups for a step of at most maxN particles (one per 4096
+// compaction tiles).
+uint32_t finalize_blocks(uint32_t maxN);
+hipError_t launch_finalize(const FinalizeArgs& a, uint32_t maxN, hipStream_t stream);
 hipError_t launch_scatter(const ScatterArgs& a, uint32_t maxN, hipStream_t stream);
 hipError_t launch_inject(const InjectArgs& a, hipStream_t stream);
 hipError_t launch_zero(const ZeroArgs& a, hipStream_t stream);
