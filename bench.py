@@ -10,8 +10,10 @@ particle-steps/s = (sum over timed steps of the particles entering the step)
 
 The default (`--config all`) measures what the headline metric names
 ("batched vs plain; p99 job latency"):
-  value            c4, the 512-replica batched ensemble (two-step hipGraphs),
-                   particle-steps/s over K timed steps after W warm-up steps;
+  value            c4, the 512-replica batched ensemble (two-step hipGraphs,
+                   grids per replica size class and launch shapes autotuned
+                   before the timed region), particle-steps/s over K timed
+                   steps after W warm-up steps;
   batched_vs_plain the same 512 replicas run one simulation per launch
                    sequence, eager and graph-captured, K steps each;
   roofline         the dominant single kernel of the c4 step;
@@ -276,9 +278,25 @@ def cpu_baseline(reps, steps, budget_s, threads):
             "cpu_model": model}
 
 
-def run_steps(cfg, args, reps=None):
-    """Engine over the workload, W warm-up steps, K timed steps; returns
-    (engine, particle-steps, seconds, description)."""
+def autotune_ensemble(reps):
+    """Grid refinement and particles per random stream per replica size class
+    (the c4 workload's autotuned cell size), untimed; returns the tuned
+    replicas and the report."""
+    from gpusmoldyn_amd import autotune
+
+    t0 = time.perf_counter()
+    choice, table = autotune.tune_ensemble(reps, deltas=(-1, 0, 1), per_stream=(8, 16))
+    tuned = autotune.apply_ensemble(reps, choice)
+    rep = {"grid_delta_and_particles_per_stream_by_size_class": {f"2^{c}": list(v) for c, v in sorted(choice.items())},
+           "ns_per_particle_step": {f"2^{c} {d:+d} /{p}": round(v, 3) for (c, d, p), v in sorted(table.items())},
+           "seconds": round(time.perf_counter() - t0, 2)}
+    return tuned, rep
+
+
+def run_steps(cfg, args, reps=None, tune=False):
+    """Engine over the workload (with tuned launch shapes when `tune`), W
+    warm-up steps, K timed steps; returns (engine, replicas, steps,
+    particle-steps, seconds, description, knob report)."""
     r, default_steps, desc = build(cfg)
     reps = reps or r
     steps = args.steps if args.steps is not None else default_steps
@@ -288,17 +306,31 @@ def run_steps(cfg, args, reps=None):
     if args.sort != 1:
         e.set_sort(args.sort)
     e.step(args.warmup)
+    knobs = None
+    if tune:
+        from gpusmoldyn_amd import autotune
+
+        t0 = time.perf_counter()
+        best, table = autotune.tune_knobs(e, steps=10)
+        knobs = {"producer_block": best[smd.KNOB_PRODUCER_BLOCK], "pair_block": best[smd.KNOB_PAIR_BLOCK],
+                 "us_per_step": {f"{a}/{b}": round(v, 1) for (a, b), v in sorted(table.items())},
+                 "seconds": round(time.perf_counter() - t0, 2)}
+        e.step(2)
     ps, secs = timed(e, steps)
-    return e, reps, steps, ps, secs, desc
+    return e, reps, steps, ps, secs, desc, knobs
 
 
-def plain_forms(reps, steps, warmup, radix_bits):
+def plain_forms(reps, steps, warmup, radix_bits, knobs=None):
     """The batched workload run one simulation per launch sequence: each
-    replica in its own engine, W warm-up steps, then K eager steps and K
-    graph-replayed steps timed separately.  Returns the totals."""
+    replica in its own engine (with the batched run's launch shapes), W
+    warm-up steps, then K eager steps and K graph-replayed steps timed
+    separately.  Returns the totals."""
     eager_ps = eager_s = graph_ps = graph_s = 0.0
     for k, rep in enumerate(reps):
         p = make_engine([rep], radix_bits)
+        if knobs:
+            p.set_knob(smd.KNOB_PRODUCER_BLOCK, knobs["producer_block"])
+            p.set_knob(smd.KNOB_PAIR_BLOCK, knobs["pair_block"])
         p.set_graphs(False)
         p.step(warmup)
         a, b = timed(p, steps)
@@ -395,6 +427,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="all: skip the c5 serving run")
     ap.add_argument("--no-c3", action="store_true", help="all: skip the c3 step")
     ap.add_argument("--no-graphs", action="store_true", help="one launch sequence per step instead of graphs")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="c4: the default grids and launch shapes instead of the autotuned ones")
     ap.add_argument("--sort", type=int, default=1, help="1: count/scan/scatter (default), 0: single-sweep radix sort")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--jobs", type=int, default=200, help="c5: number of jobs")
@@ -417,15 +451,23 @@ def main():
     if args.config == "all" and args.steps is None:
         args.steps = 20
     log(f"{cfg}: building the workload")
-    e, reps, steps, ps, secs, desc = run_steps(cfg, args)
+    tuned, tune_rep = None, None
+    tune = cfg == "c4" and not args.no_autotune
+    if tune:
+        tuned, tune_rep = autotune_ensemble(build(cfg)[0])
+        log(f"{cfg}: tuned {tune_rep['grid_delta_and_particles_per_stream_by_size_class']} in "
+            f"{tune_rep['seconds']} s")
+    e, reps, steps, ps, secs, desc, knobs = run_steps(cfg, args, reps=tuned, tune=tune)
     log(f"{cfg}: {steps} steps in {secs * 1e3:.2f} ms")
     out = base_line(cfg, args, e, steps, ps, secs, desc)
     e.close()
+    if tune:
+        out["autotune"] = dict(tune_rep, launch=knobs)
     if probe:
         out["copy_probe"] = probe
     if cfg == "c4" and not args.no_plain:
         log("c4: plain forms (each replica alone)")
-        eps, es, gps, gs = plain_forms(reps, steps, args.warmup, args.radix_bits)
+        eps, es, gps, gs = plain_forms(reps, steps, args.warmup, args.radix_bits, knobs)
         out["batched_vs_plain"] = {
             "batched": out["value"], "plain_eager": eps / es, "plain_graph": gps / gs,
             "ratio_vs_eager": out["value"] / (eps / es), "ratio_vs_graph": out["value"] / (gps / gs),
@@ -434,7 +476,7 @@ def main():
                     "hipGraphs (graph), K steps each after the same warm-up"}
     if args.config == "all" and not args.no_c3:
         log("c3: 4M-particle clustered step")
-        e3, _, s3, ps3, secs3, d3 = run_steps("c3", args)
+        e3, _, s3, ps3, secs3, d3, _ = run_steps("c3", args)
         l3 = base_line("c3", args, e3, s3, ps3, secs3, d3)
         e3.close()
         out["c3_4M"] = {"workload": d3["workload"], "steps": s3, "ms_per_step": l3["ms_per_step"],

@@ -20,6 +20,8 @@ MT_PARAM_SETS = 32768
 GAUSS_DIM = 4096
 
 BC_CLAMP, BC_REFLECT, BC_ABSORB, BC_PERIODIC = 0, 1, 2, 3
+KNOB_PRODUCER_BLOCK, KNOB_PAIR_BLOCK = 0, 1
+KNOB_VALUES = {KNOB_PRODUCER_BLOCK: (128, 256, 512), KNOB_PAIR_BLOCK: (64, 128, 256)}
 
 
 class SmdError(RuntimeError):
@@ -98,6 +100,8 @@ def lib():
         L.smd_engine_particle_steps.argtypes = [vp, ctypes.POINTER(ctypes.c_ulonglong)]
         L.smd_engine_set_graphs.argtypes = [vp, ctypes.c_int]
         L.smd_engine_set_sort.argtypes = [vp, ctypes.c_int]
+        L.smd_engine_set_knob.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.smd_engine_get_knob.argtypes = [vp, ctypes.c_int]
         L.smd_engine_num_replicas.argtypes = [vp]
         L.smd_engine_radix_bits.argtypes = [vp]
         L.smd_engine_num_replicas.restype = ctypes.c_uint
@@ -294,6 +298,17 @@ class Engine:
     def set_sort(self, kind):
         """1: count / scan / scatter per digit (default), 0: single-sweep radix sort."""
         _check(lib().smd_engine_set_sort(self._h, int(kind)), "smd_engine_set_sort")
+
+    def set_knob(self, knob, value):
+        """Launch shape (KNOB_PRODUCER_BLOCK, KNOB_PAIR_BLOCK); results are
+        identical for every setting."""
+        _check(lib().smd_engine_set_knob(self._h, int(knob), int(value)), "smd_engine_set_knob")
+
+    def get_knob(self, knob):
+        v = lib().smd_engine_get_knob(self._h, int(knob))
+        if v < 0:
+            _check(-v, "smd_engine_get_knob")
+        return v
 
     def set_graphs(self, on):
         _check(lib().smd_engine_set_graphs(self._h, int(bool(on))), "smd_engine_set_graphs")

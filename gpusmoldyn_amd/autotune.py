@@ -1,14 +1,18 @@
-"""Grid and sort-digit autotuner.
+"""Autotuner: hash grid, sort digit width and kernel launch shapes.
 
 The hash grid (cells per axis, a power of two; the cell size follows from
-the box) and the sort digit width do not change which particles react: a
-pair reacts when each is the other's nearest reactive partner within the
-binding radius, whatever the bucketing.  They do change the sorted order,
-hence which stream serves a pair's draws (sorted index j uses stream j mod S),
-so a tuned run is statistically, not bitwise, equal to an untuned one.
+the box) does not change which particles react: a pair reacts when each is
+the other's nearest reactive partner within the binding radius, whatever the
+bucketing.  It does change the sorted order, hence which stream serves a
+pair's draws (sorted index j uses stream j mod S), so a run on a tuned grid is
+statistically, not bitwise, equal to an untuned one (and bitwise equal to the
+oracle on the same grid).  The sort digit width and the launch shapes
+(workgroup sizes of the producer and of the pair search) change no result.
 
-tune(spec, pos, types) times a few steps of each candidate on the GPU and
-returns the fastest (grid_bits, radix_bits) with the timing table.
+tune(spec, pos, types)    grid and digit width of one simulation
+tune_knobs(engine)        launch shapes of a finalized engine, in place
+tune_ensemble(reps)       grid refinement per replica size class of an
+                          ensemble; apply_ensemble(reps, deltas) applies it
 """
 import copy
 import time
@@ -81,3 +85,93 @@ def apply(spec, grid_bits, cell=None):
         if cell is not None:
             s.cell_size[d] = cell
     return s
+
+
+# ---------------------------------------------------------------- launch shapes
+def tune_knobs(engine, steps=10, values=None):
+    """Time `steps` steps of `engine` (finalized, warmed up) under every
+    combination of launch shapes and leave the fastest set.  The shapes do
+    not change any result, so the timing runs continue one simulation.
+    Returns (best {knob: value}, table {(producer, pair): us per step})."""
+    from . import KNOB_PAIR_BLOCK, KNOB_PRODUCER_BLOCK, KNOB_VALUES
+
+    values = values or KNOB_VALUES
+    table = {}
+    for pb in values[KNOB_PRODUCER_BLOCK]:
+        for qb in values[KNOB_PAIR_BLOCK]:
+            engine.set_knob(KNOB_PRODUCER_BLOCK, pb)
+            engine.set_knob(KNOB_PAIR_BLOCK, qb)
+            engine.step(2)  # capture the step graphs with this shape
+            engine.sync()
+            t0 = time.perf_counter()
+            engine.step(steps)
+            engine.sync()
+            table[(pb, qb)] = (time.perf_counter() - t0) * 1e6 / steps
+    best = min(table, key=table.get)
+    engine.set_knob(KNOB_PRODUCER_BLOCK, best[0])
+    engine.set_knob(KNOB_PAIR_BLOCK, best[1])
+    return {KNOB_PRODUCER_BLOCK: best[0], KNOB_PAIR_BLOCK: best[1]}, table
+
+
+# ---------------------------------------------------------------- ensembles
+def size_class(n):
+    """Replicas are tuned per power-of-two size class (n in [2^c, 2^(c+1)))."""
+    return max(0, int(n).bit_length() - 1)
+
+
+def _shift(spec, delta, per_stream=None, n=None):
+    s = apply(spec, tuple(int(spec.grid_bits[d]) + delta for d in range(3)))
+    if per_stream:
+        s.num_streams = max(64, min(32768, -(-int(n) // int(per_stream))))
+    return s
+
+
+def tune_ensemble(reps, deltas=(-1, 0, 1), per_stream=(None,), per_class=24, steps=10, warmup=3, device=0):
+    """Per size class, the grid refinement (delta bits per axis on each
+    replica's own grid) and the particles per random stream (None: the
+    replica's own stream count) that step a sample of that class's replicas
+    fastest per particle.  reps: [(spec, pos, types)].  Returns
+    ({class: (delta, per_stream)}, {(class, delta, per_stream): ns per
+    particle-step})."""
+    classes = {}
+    for k, (_, _, types) in enumerate(reps):
+        classes.setdefault(size_class(len(types)), []).append(k)
+    table = {}
+    best = {}
+    for c, idx in sorted(classes.items()):
+        sample = idx[:: max(1, len(idx) // per_class)][:per_class]
+        for dlt in deltas:
+            for per in per_stream:
+                e = Engine(device)
+                try:
+                    for k in sample:
+                        spec, pos, types = reps[k]
+                        e.add_replica(_shift(spec, dlt, per, len(types)), pos, types)
+                    e.finalize()
+                except SmdError:
+                    e.close()
+                    continue
+                try:
+                    e.step(warmup)
+                    e.sync()
+                    ps0 = e.particle_steps()
+                    t0 = time.perf_counter()
+                    e.step(steps)
+                    e.sync()
+                    table[(c, dlt, per)] = (time.perf_counter() - t0) * 1e9 / max(e.particle_steps() - ps0, 1)
+                finally:
+                    e.close()
+        cands = {(d, p): table[(c, d, p)] for d in deltas for p in per_stream if (c, d, p) in table}
+        best[c] = min(cands, key=cands.get) if cands else (0, None)
+    return best, table
+
+
+def apply_ensemble(reps, choice):
+    """The replicas with each one's grid refined (and stream count set) by its
+    size class's choice: {class: (delta, per_stream)} or {class: delta}."""
+    out = []
+    for spec, pos, types in reps:
+        ch = choice.get(size_class(len(types)), 0)
+        dlt, per = ch if isinstance(ch, tuple) else (ch, None)
+        out.append((_shift(spec, dlt, per, len(types)), pos, types))
+    return out

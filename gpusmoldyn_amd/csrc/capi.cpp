@@ -247,6 +247,18 @@ smd_status smd_engine_set_graphs(smd_engine* e, int on) {
   return SMD_OK;
 }
 
+smd_status smd_engine_set_knob(smd_engine* e, int knob, int value) {
+  if (!e) return SMD_ERR_INVALID;
+  return e->impl->set_knob(knob, value);
+}
+
+int smd_engine_get_knob(const smd_engine* e, int knob) {
+  if (!e) return -SMD_ERR_INVALID;
+  if (knob == SMD_KNOB_PRODUCER_BLOCK) return e->impl->producerBlock;
+  if (knob == SMD_KNOB_PAIR_BLOCK) return e->impl->pairBlock;
+  return -SMD_ERR_INVALID;
+}
+
 smd_status smd_engine_set_sort(smd_engine* e, int kind) {
   if (!e || (kind != 0 && kind != 1)) return SMD_ERR_INVALID;
   e->impl->drop_graph();

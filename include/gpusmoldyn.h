@@ -178,6 +178,17 @@ smd_status smd_engine_step(smd_engine* e, int nsteps);
 /* Replay pairs of steps as one captured hipGraph (default on); off = one
  * launch sequence per step. */
 smd_status smd_engine_set_graphs(smd_engine* e, int on);
+/* Launch shapes of the step kernels, for the autotuner: every setting gives
+ * the same results bit for bit.  SMD_KNOB_PRODUCER_BLOCK: threads per
+ * workgroup of the diffuse/react/hash kernel (128, 256 = default, 512; one
+ * thread per random stream); SMD_KNOB_PAIR_BLOCK: of the pair search (64,
+ * 128, 256 = default; one thread per sorted particle).  No counterpart in
+ * the reference (fixed 256-thread diffuse and 64-thread collide blocks,
+ * particleSystemCuda.cu:167, 336). */
+typedef enum smd_knob { SMD_KNOB_PRODUCER_BLOCK = 0, SMD_KNOB_PAIR_BLOCK = 1 } smd_knob;
+smd_status smd_engine_set_knob(smd_engine* e, int knob, int value);
+/* Current value of a knob, or -SMD_ERR_INVALID. */
+int smd_engine_get_knob(const smd_engine* e, int knob);
 /* Key sort of the step: 1 = count, scan and scatter kernels per digit
  * (default), 0 = single sweep (one histogram kernel, one binning kernel per
  * digit with decoupled look-back).  Both give the same stable order. */

@@ -92,14 +92,16 @@ def test_c4_all_replicas(smd, orc, gpu):
 
 
 def test_c5_jobs_match_oracle(smd, orc, gpu):
-    """Jobs of the serving stream itself (its first jobs below 1e7
-    particle-steps: 2, 3 and 4 species), served together through the slot
-    engine at the serving layout (32 slots of 2^18 buckets), each equal to
-    the oracle running the job alone."""
+    """Jobs of the serving stream itself (of each species count 2, 3 and 4,
+    its first three below 1e7 particle-steps), served together through the
+    slot engine at the serving layout (32 slots of 2^18 buckets), each equal
+    to the oracle running the job alone."""
     from gpusmoldyn_amd import serving
 
-    jobs = [j for j in serving.c5_jobs() if len(j.types) * j.steps <= 1e7][:8]
-    assert {j.spec.num_types - 1 for j in jobs} == {2, 3, 4}
+    small = [j for j in serving.c5_jobs() if len(j.types) * j.steps <= 1e7]
+    jobs = [j for sp in (2, 3, 4) for j in [x for x in small if x.spec.num_types - 1 == sp][:3]]
+    jobs.sort(key=lambda j: j.arrival)
+    assert {j.spec.num_types - 1 for j in jobs} == {2, 3, 4} and len(jobs) == 9
     banner(f"c5 jobs {[(j.jid, len(j.types), j.steps, j.spec.num_types - 1) for j in jobs]}")
     srv = serving.Server(n_slots=32, slot_bucket_bits=18, slot_streams=32768, capacity=1 << 22,
                          inject_cap=1 << 20)

@@ -40,7 +40,7 @@ def test_mt_matches_reference_dcmt(smd, gpu):
 @pytest.mark.parametrize("n,bits", [(0, 8), (1, 8), (4095, 12), (4096, 12), (4097, 20), (100003, 24), (1 << 20, 21),
                                     (3000000, 25)])
 def test_sort_pairs_stable(smd, gpu, n, bits, radix_bits):
-    # up to 12.6M keys the sort runs 2048-key tiles, beyond that 4096-key tiles
+    # up to 6M keys the sort runs 2048-key tiles, beyond that 4096-key tiles
     banner(f"sort n={n} bits={bits} radix={radix_bits}")
     rng = np.random.default_rng(n + bits)
     keys = rng.integers(0, 1 << bits, size=n, dtype=np.uint64).astype(np.uint32)
@@ -54,7 +54,9 @@ def test_sort_pairs_stable(smd, gpu, n, bits, radix_bits):
 
 
 def test_sort_pairs_large_tiles(smd, gpu):
-    # above 12.6M keys: 4096-key tiles, ragged last tile
+    # above 6M keys: 4096-key tiles, ragged last tile (the engine's own cases,
+    # where the expected count and the bound pick different tile sizes, run in
+    # the full-size c3 and c4 tests)
     n, bits = 13_000_001, 23
     banner(f"sort n={n} bits={bits} radix=8")
     rng = np.random.default_rng(7)
@@ -382,3 +384,52 @@ def test_sort_kinds_agree(smd, gpu):
     ap, at = a.get(0)
     bp, bt = b.get(0)
     assert np.array_equal(at, bt) and np.array_equal(ap.view(np.uint32), bp.view(np.uint32))
+
+
+def test_launch_shapes_agree(smd, gpu):
+    """Every producer and pair-search workgroup size gives the same particles,
+    bit for bit (the autotuner picks among them by time alone)."""
+    spec, pos, types = w_c3_small()
+    banner("launch shapes")
+    res = []
+    for pb, qb in ((256, 256), (128, 64), (512, 128), (512, 64)):
+        e = _engine(smd, spec, pos, types)
+        e.set_knob(smd.KNOB_PRODUCER_BLOCK, pb)
+        e.set_knob(smd.KNOB_PAIR_BLOCK, qb)
+        assert e.get_knob(smd.KNOB_PRODUCER_BLOCK) == pb and e.get_knob(smd.KNOB_PAIR_BLOCK) == qb
+        e.step(6)
+        res.append(e.get(0))
+        e.close()
+    for p, t in res[1:]:
+        assert np.array_equal(res[0][1], t) and np.array_equal(res[0][0].view(np.uint32), p.view(np.uint32))
+    with pytest.raises(smd.SmdError):
+        e = _engine(smd, spec, pos, types)
+        try:
+            e.set_knob(smd.KNOB_PAIR_BLOCK, 96)
+        finally:
+            e.close()
+
+
+def test_tuned_ensemble_matches_oracle(smd, orc, gpu):
+    """The per-size-class choices of the autotuner are valid simulations:
+    replicas on refined and coarsened grids, with their own or fewer streams,
+    equal the oracle on the same grids and streams."""
+    from gpusmoldyn_amd import autotune
+
+    reps = configs.c4(n_rep=12, nmin=2048, nmax=16384)
+    deltas = {autotune.size_class(len(t)): ((-1, 16), (1, None))[k % 2] for k, (_, _, t) in enumerate(reps)}
+    tuned = autotune.apply_ensemble(reps, deltas)
+    banner(f"tuned ensemble deltas {deltas}")
+    e = smd.Engine(0)
+    for spec, pos, types in tuned:
+        e.add_replica(spec, pos, types)
+    e.finalize()
+    e.step(8)
+    g = smd.gauss_table()
+    for r, (spec, pos, types) in enumerate(tuned):
+        sim = orc.sim_for(smd, spec, pos, types)
+        sim.step(g, 8)
+        gp, gt = e.get(r)
+        rp, rt = sim.state()
+        assert np.array_equal(gt, rt) and np.array_equal(gp.view(np.uint32), rp.view(np.uint32)), r
+    e.close()
