@@ -147,18 +147,21 @@ def stage_graph_bytes(e, steps, ps):
     return n * (140 + 16 * P) + 12 * lay["total_buckets"] + 336 * lay["total_streams"]
 
 
-def roofline(e, cfg, nprof=5):
+def roofline(e, cfg, nprof=5, streams=None, buckets=None):
     """Per-stage HIP-event times of `nprof` eager steps (on the engine's
-    stream) and the dominant single kernel's roofline."""
+    stream) and the dominant single kernel's roofline.  streams / buckets:
+    the streams and buckets in use when they are fewer than the engine's
+    layout (a slot engine's idle slots)."""
     prof = e.profile(nprof)
     lay = e.layout()
     n = int(e.counts().sum())
-    S = lay["total_streams"]
+    S = streams if streams is not None else lay["total_streams"]
+    nb = buckets if buckets is not None else lay["total_buckets"]
     rb = e.radix_bits()
     passes = (lay["key_bits"] + rb - 1) // rb
     rows = {}
     for k, us in prof.items():
-        b = stage_bytes(k, n, S, lay["total_buckets"], passes)
+        b = stage_bytes(k, n, S, nb, passes)
         rows[k] = dict(us=us, bytes=b, gbs=(b / (us * 1e-6) / 1e9) if us > 0 else 0.0)
     dom = max((k for k in rows if k in SINGLE_KERNEL), key=lambda k: rows[k]["us"])
     step_us = sum(r["us"] for r in rows.values())
@@ -278,22 +281,30 @@ def cpu_baseline(reps, steps, budget_s, threads):
             "cpu_model": model}
 
 
-def autotune_ensemble(reps):
+def autotune_ensemble(reps, path=None):
     """Grid refinement and particles per random stream per replica size class
     (the c4 workload's autotuned cell size), untimed; returns the tuned
-    replicas and the report."""
+    replicas and the report.  path: a JSON file of an earlier tuning to reuse
+    (written when absent), so profiling runs step the same workload."""
     from gpusmoldyn_amd import autotune
 
     t0 = time.perf_counter()
-    choice, table = autotune.tune_ensemble(reps, deltas=(-1, 0, 1), per_stream=(8, 16))
+    if path and os.path.exists(path):
+        saved = json.load(open(path))
+        choice = {int(c): (d, p) for c, (d, p) in saved["choice"].items()}
+        table = {}
+    else:
+        choice, table = autotune.tune_ensemble(reps, deltas=(-1, 0, 1), per_stream=(8, 16))
+        if path:
+            json.dump({"choice": {str(c): list(v) for c, v in choice.items()}}, open(path, "w"))
     tuned = autotune.apply_ensemble(reps, choice)
     rep = {"grid_delta_and_particles_per_stream_by_size_class": {f"2^{c}": list(v) for c, v in sorted(choice.items())},
-           "ns_per_particle_step": {f"2^{c} {d:+d} /{p}": round(v, 3) for (c, d, p), v in sorted(table.items())},
+           "ns_per_particle_step": {k: round(v, 4) for k, v in table.items()},
            "seconds": round(time.perf_counter() - t0, 2)}
     return tuned, rep
 
 
-def run_steps(cfg, args, reps=None, tune=False):
+def run_steps(cfg, args, reps=None, tune=False, knob_file=None):
     """Engine over the workload (with tuned launch shapes when `tune`), W
     warm-up steps, K timed steps; returns (engine, replicas, steps,
     particle-steps, seconds, description, knob report)."""
@@ -311,10 +322,20 @@ def run_steps(cfg, args, reps=None, tune=False):
         from gpusmoldyn_amd import autotune
 
         t0 = time.perf_counter()
-        best, table = autotune.tune_knobs(e, steps=10)
+        if knob_file and os.path.exists(knob_file):
+            saved = json.load(open(knob_file))
+            best = {smd.KNOB_PRODUCER_BLOCK: saved["producer_block"], smd.KNOB_PAIR_BLOCK: saved["pair_block"]}
+            for k, v in best.items():
+                e.set_knob(k, v)
+            table = {}
+        else:
+            best, table = autotune.tune_knobs(e, steps=10)
         knobs = {"producer_block": best[smd.KNOB_PRODUCER_BLOCK], "pair_block": best[smd.KNOB_PAIR_BLOCK],
                  "us_per_step": {f"{a}/{b}": round(v, 1) for (a, b), v in sorted(table.items())},
                  "seconds": round(time.perf_counter() - t0, 2)}
+        if knob_file and not os.path.exists(knob_file):
+            json.dump({"producer_block": knobs["producer_block"], "pair_block": knobs["pair_block"]},
+                      open(knob_file, "w"))
         e.step(2)
     ps, secs = timed(e, steps)
     return e, reps, steps, ps, secs, desc, knobs
@@ -359,6 +380,27 @@ def serve(args):
     return jobs, st
 
 
+def serving_roofline(jobs, resident=8, nprof=5):
+    """Stage times (HIP events) and the dominant kernel's roofline of the
+    serving engine with a typical resident mix: the stream's first `resident`
+    jobs admitted together into the serving layout, stepped past their entry
+    step, then `nprof` eager steps timed."""
+    from gpusmoldyn_amd import serving
+
+    srv = serving.Server(n_slots=32, slot_bucket_bits=18, slot_streams=32768, capacity=1 << 24)
+    e = srv.e
+    mix = jobs[:resident]
+    for k, j in enumerate(mix):
+        e.admit(k, j.spec, j.pos, j.types)
+    e.step(4)
+    streams = sum(int(j.spec.num_streams) for j in mix)
+    buckets = sum(1 << sum(int(j.spec.grid_bits[d]) for d in range(3)) for j in mix)
+    rl, step = roofline(e, "c5", nprof, streams=streams, buckets=buckets)
+    step["resident_particles"] = int(e.counts().sum())
+    srv.close()
+    return rl, step
+
+
 def serving_report(jobs, st):
     return {"p50_latency_s": round(st["p50_s"], 4), "p99_latency_s": round(st["p99_s"], 4),
             "particle_steps_per_s": st["particle_steps_per_s"], "wall_s": round(st["wall_s"], 2),
@@ -370,7 +412,7 @@ def serving_report(jobs, st):
 
 def c5_line(args):
     jobs, st = serve(args)
-    return {
+    out = {
         "metric": METRIC,
         "value": st["particle_steps_per_s"],
         "unit": "particle-steps/s",
@@ -386,8 +428,9 @@ def c5_line(args):
         "config": {"workload": serving_report(jobs, st)["workload"], "jobs": len(jobs),
                    "particle_steps": st["particle_steps"]},
         "serving": serving_report(jobs, st),
-        "roofline": None,
     }
+    out["roofline"], out["step_roofline"] = serving_roofline(jobs)
+    return out
 
 
 def base_line(cfg, args, e, steps, ps, secs, desc):
@@ -429,6 +472,9 @@ def main():
     ap.add_argument("--no-graphs", action="store_true", help="one launch sequence per step instead of graphs")
     ap.add_argument("--no-autotune", action="store_true",
                     help="c4: the default grids and launch shapes instead of the autotuned ones")
+    ap.add_argument("--tuning", default=None, metavar="PREFIX",
+                    help="c4: reuse the tuning saved at PREFIX.grid.json / PREFIX.launch.json (saved there "
+                         "when absent), so that profiling runs step the benchmark's workload")
     ap.add_argument("--sort", type=int, default=1, help="1: count/scan/scatter (default), 0: single-sweep radix sort")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--jobs", type=int, default=200, help="c5: number of jobs")
@@ -454,10 +500,11 @@ def main():
     tuned, tune_rep = None, None
     tune = cfg == "c4" and not args.no_autotune
     if tune:
-        tuned, tune_rep = autotune_ensemble(build(cfg)[0])
+        tuned, tune_rep = autotune_ensemble(build(cfg)[0], args.tuning + ".grid.json" if args.tuning else None)
         log(f"{cfg}: tuned {tune_rep['grid_delta_and_particles_per_stream_by_size_class']} in "
             f"{tune_rep['seconds']} s")
-    e, reps, steps, ps, secs, desc, knobs = run_steps(cfg, args, reps=tuned, tune=tune)
+    e, reps, steps, ps, secs, desc, knobs = run_steps(cfg, args, reps=tuned, tune=tune,
+                                                      knob_file=args.tuning + ".launch.json" if args.tuning else None)
     log(f"{cfg}: {steps} steps in {secs * 1e3:.2f} ms")
     out = base_line(cfg, args, e, steps, ps, secs, desc)
     e.close()
@@ -486,6 +533,9 @@ def main():
         log("c5: serving stream")
         jobs, st = serve(args)
         out["serving"] = serving_report(jobs, st)
+        rl5, st5 = serving_roofline(jobs)
+        out["serving"]["roofline"] = rl5
+        out["serving"]["stages_us"] = st5["stages_us"]
     if not args.no_cpu:
         log("CPU baseline")
         threads = host_cores()

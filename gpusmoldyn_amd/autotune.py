@@ -126,43 +126,64 @@ def _shift(spec, delta, per_stream=None, n=None):
     return s
 
 
-def tune_ensemble(reps, deltas=(-1, 0, 1), per_stream=(None,), per_class=24, steps=10, warmup=3, device=0):
+def _time_ensemble(reps, choice, steps, warmup, device):
+    """Wall time per particle-step (ns) of the whole ensemble with the
+    per-class choice applied; None when the engine rejects it."""
+    e = Engine(device)
+    try:
+        for spec, pos, types in apply_ensemble(reps, choice):
+            e.add_replica(spec, pos, types)
+        e.finalize()
+    except SmdError:
+        e.close()
+        return None
+    try:
+        e.step(warmup)
+        e.sync()
+        ps0 = e.particle_steps()
+        t0 = time.perf_counter()
+        e.step(steps)
+        e.sync()
+        return (time.perf_counter() - t0) * 1e9 / max(e.particle_steps() - ps0, 1)
+    finally:
+        e.close()
+
+
+def tune_ensemble(reps, deltas=(-1, 0, 1), per_stream=(None,), steps=10, warmup=4, device=0):
     """Per size class, the grid refinement (delta bits per axis on each
     replica's own grid) and the particles per random stream (None: the
-    replica's own stream count) that step a sample of that class's replicas
-    fastest per particle.  reps: [(spec, pos, types)].  Returns
-    ({class: (delta, per_stream)}, {(class, delta, per_stream): ns per
-    particle-step})."""
-    classes = {}
-    for k, (_, _, types) in enumerate(reps):
-        classes.setdefault(size_class(len(types)), []).append(k)
+    replica's own stream count), timed on the whole ensemble: the key width
+    of the sort and the size of the cell table depend on every replica's
+    grid together, so classes are not timed apart.  First one choice for
+    every class, then each class in turn (largest first) tries the other
+    refinements with the rest fixed.  reps: [(spec, pos, types)].  Returns
+    ({class: (delta, per_stream)}, {label: ns per particle-step})."""
+    classes = sorted({size_class(len(t)) for _, _, t in reps}, reverse=True)
     table = {}
-    best = {}
-    for c, idx in sorted(classes.items()):
-        sample = idx[:: max(1, len(idx) // per_class)][:per_class]
-        for dlt in deltas:
-            for per in per_stream:
-                e = Engine(device)
-                try:
-                    for k in sample:
-                        spec, pos, types = reps[k]
-                        e.add_replica(_shift(spec, dlt, per, len(types)), pos, types)
-                    e.finalize()
-                except SmdError:
-                    e.close()
-                    continue
-                try:
-                    e.step(warmup)
-                    e.sync()
-                    ps0 = e.particle_steps()
-                    t0 = time.perf_counter()
-                    e.step(steps)
-                    e.sync()
-                    table[(c, dlt, per)] = (time.perf_counter() - t0) * 1e9 / max(e.particle_steps() - ps0, 1)
-                finally:
-                    e.close()
-        cands = {(d, p): table[(c, d, p)] for d in deltas for p in per_stream if (c, d, p) in table}
-        best[c] = min(cands, key=cands.get) if cands else (0, None)
+    best, best_t = None, None
+    for d in deltas:
+        for p in per_stream:
+            ch = {c: (d, p) for c in classes}
+            t = _time_ensemble(reps, ch, steps, warmup, device)
+            if t is None:
+                continue
+            table[f"all {d:+d} /{p}"] = t
+            if best_t is None or t < best_t:
+                best, best_t = ch, t
+    if best is None:
+        raise SmdError("no valid ensemble grid")
+    for c in classes:
+        for d in deltas:
+            if d == best[c][0]:
+                continue
+            ch = dict(best)
+            ch[c] = (d, best[c][1])
+            t = _time_ensemble(reps, ch, steps, warmup, device)
+            if t is None:
+                continue
+            table[f"2^{c} {d:+d}"] = t
+            if t < best_t:
+                best, best_t = ch, t
     return best, table
 
 
