@@ -24,7 +24,7 @@ HIP_FLAGS = [
     "-Wno-unused-function",
 ]
 
-SOURCES = ["kernels.hip", "legacy.hip", "copy_probe.hip", "engine.cpp", "setup.cpp", "rdf.cpp", "rdf_gpu.hip", "capi.cpp", "particle_system.cpp", "sfmt.cpp"]
+SOURCES = ["kernels.hip", "legacy.hip", "copy_probe.hip", "engine.cpp", "setup.cpp", "rdf.cpp", "rdf_gpu.hip", "capi.cpp", "particle_system.cpp", "sfmt.cpp", "noalloc.hip"]
 
 
 def hipcc():

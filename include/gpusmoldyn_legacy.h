@@ -163,6 +163,16 @@ int radixSort(unsigned int* keys, unsigned int* values, unsigned int* tempKeys, 
               unsigned int* counters, unsigned int* countersSum, unsigned int* blockOffsets, void* scanPlan,
               unsigned int numElements, unsigned int keyBits, bool flipBits);
 
+/* No counterpart in the reference: radixSort for callers that cannot
+ * allocate device memory (stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same sort, same arguments minus the unused ones; the
+ * digit counters live in a fixed device array of the library (about 4 MiB per
+ * GPU) instead of a buffer it allocates, so numElements is limited to 2^24
+ * (larger: SMD_ERR_CAPACITY, nothing launched).  Calls that overlap on one
+ * GPU share that array and race. */
+int smdRadixSortNoAlloc(unsigned int* keys, unsigned int* values, unsigned int* tempKeys, unsigned int* tempValues,
+                        unsigned int numElements, unsigned int keyBits);
+
 /* Exclusive + scan of n ints on the device (out may alias in). */
 int smdExclusiveScanInt(int* out, const int* in, unsigned int n);
 
