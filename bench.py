@@ -167,8 +167,14 @@ def roofline(e, cfg, nprof=5, streams=None, buckets=None):
     step_us = sum(r["us"] for r in rows.values())
     step_bytes = sum(r["bytes"] for r in rows.values())
     ach = rows[dom]["gbs"]
-    traffic, prof_us, prof_file = profiled_traffic(cfg, SINGLE_KERNEL[dom])
-    rl = {"bound": "hbm", "kernel": SINGLE_KERNEL[dom], "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+    # the kernels with a launch-shape knob are templates on their block size
+    kname = SINGLE_KERNEL[dom]
+    if dom == "producer":
+        kname += f"<{e.get_knob(smd.KNOB_PRODUCER_BLOCK)}>"
+    elif dom == "candidates":
+        kname += f"<{e.get_knob(smd.KNOB_PAIR_BLOCK)}>"
+    traffic, prof_us, prof_file = profiled_traffic(cfg, kname)
+    rl = {"bound": "hbm", "kernel": kname, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
           "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
           "bytes_per_launch": rows[dom]["bytes"], "avg_us": round(rows[dom]["us"], 2),
           "profile": prof_file, "profile_avg_us": prof_us}
@@ -294,7 +300,7 @@ def autotune_ensemble(reps, path=None):
         choice = {int(c): (d, p) for c, (d, p) in saved["choice"].items()}
         table = {}
     else:
-        choice, table = autotune.tune_ensemble(reps, deltas=(-1, 0, 1), per_stream=(8, 16))
+        choice, table = autotune.tune_ensemble(reps, deltas=(-1, 0, 1), per_stream=(8, 16, 32))
         if path:
             json.dump({"choice": {str(c): list(v) for c, v in choice.items()}}, open(path, "w"))
     tuned = autotune.apply_ensemble(reps, choice)
@@ -372,7 +378,8 @@ def serve(args):
     from gpusmoldyn_amd import serving
 
     jobs = serving.c5_jobs(n_jobs=args.jobs)
-    srv = serving.Server(n_slots=32, slot_bucket_bits=18, slot_streams=32768, capacity=1 << 24)
+    srv = serving.Server(n_slots=32, slot_bucket_bits=18, slot_streams=32768, capacity=1 << 24,
+                         graphs=not args.no_graphs)
     # warm-up: one small job through the same slots
     srv.run([serving.make_job(-1, 0.0, 4096, 20, 3, 1)], realtime=False)
     results, st = srv.run(jobs, realtime=True)
