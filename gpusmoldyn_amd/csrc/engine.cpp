@@ -92,6 +92,7 @@ Engine::~Engine() {
   dfree(dTables);
   dfree(dStreamReplica);
   dfree(dBucketStart);
+  dfree(dBucketRep);
   dfree(dRepOff);
   dfree(dRepCount);
   dfree(dNewOff);
@@ -265,6 +266,28 @@ smd_status Engine::finalize(uint32_t capacity) {
   bstart[R] = deadKey;
   HIPCHK(dalloc(&dBucketStart, R + 1));
   HIPCHK(hipMemcpy(dBucketStart, bstart.data(), (R + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
+  // Replica of a bucket by one lookup: every replica owns a power-of-two run
+  // of buckets at an offset that is a multiple of the smallest such run, so
+  // chunks of that size never straddle two replicas.
+  {
+    uint32_t minBits = 31;
+    for (uint32_t r = 0; r < R; ++r) {
+      uint32_t b = 0;
+      while ((1u << b) < tabs[r].numBuckets) ++b;
+      minBits = std::min(minBits, b);
+    }
+    const uint64_t chunks = ((uint64_t)deadKey + (1ull << minBits) - 1) >> minBits;
+    if (R > 1 && chunks <= (1ull << 22)) {
+      std::vector<int> rep((size_t)std::max<uint64_t>(chunks, 1), -1);
+      for (uint32_t r = 0; r < R; ++r)
+        for (uint64_t c = tabs[r].bucketOff >> minBits; c < ((uint64_t)tabs[r].bucketOff + tabs[r].numBuckets) >> minBits;
+             ++c)
+          rep[c] = (int)r;
+      HIPCHK(dalloc(&dBucketRep, rep.size()));
+      HIPCHK(hipMemcpy(dBucketRep, rep.data(), rep.size() * sizeof(int), hipMemcpyHostToDevice));
+      repShift = minBits;
+    }
+  }
   std::vector<uint32_t> off(R + 1), cnt(R);
   uint64_t o = 0;
   for (uint32_t r = 0; r < R; ++r) {
@@ -599,6 +622,8 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
     CandArgs ca;
     ca.tables = dTables;
     ca.bucketStart = dBucketStart;
+    ca.bucketRep = dBucketRep;
+    ca.repShift = repShift;
     ca.R = (int)R;
     ca.keys = sb.keys;
     ca.sortedPT = dSortedPT;
@@ -1087,6 +1112,8 @@ smd_status Engine::debug_stage(int stage, float* pos4, int* types, uint32_t* key
     CandArgs ca;
     ca.tables = dTables;
     ca.bucketStart = dBucketStart;
+    ca.bucketRep = dBucketRep;
+    ca.repShift = repShift;
     ca.R = (int)R;
     ca.keys = sb.keys;
     ca.sortedPT = dSortedPT;

@@ -300,9 +300,21 @@ __global__ __launch_bounds__(256) void k_sort_up(const uint32_t* __restrict__ ke
   }
   for (int d = threadIdx.x; d < RADIX; d += 256) hist[d] = 0;
   __syncthreads();
+  // A chunk of 64 keys whose digits agree (the high digits of cell-ordered
+  // or replica-ordered keys mostly do) adds its count with one LDS atomic
+  // instead of 64 on the same address.
 #pragma unroll
-  for (int k = 0; k < ITEMS; ++k)
-    if (base + k * 256 + threadIdx.x < n) atomicAdd(&hist[(kk[k] >> shift) & (RADIX - 1)], 1u);
+  for (int k = 0; k < ITEMS; ++k) {
+    const bool valid = base + k * 256 + threadIdx.x < n;
+    const uint32_t d = (kk[k] >> shift) & (RADIX - 1);
+    const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+    const uint64_t vm = __ballot(valid);
+    if (__all(d == d0 || !valid)) {
+      if ((threadIdx.x & 63u) == 0u && vm) atomicAdd(&hist[d0], (uint32_t)__popcll(vm));
+    } else if (valid) {
+      atomicAdd(&hist[d], 1u);
+    }
+  }
   __syncthreads();
   for (int d = threadIdx.x; d < RADIX; d += 256) counts[(size_t)d * stride + tile] = hist[d];
 }
@@ -1063,6 +1075,7 @@ __device__ __forceinline__ void wave_search(const ReplicaTables& T, const CandAr
 // table fields load through the scalar cache.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_candidates(CandArgs a, const uint32_t* __restrict__ bucketStart,
+                                                   const int* __restrict__ bucketRep,
                                                    const ReplicaTables* __restrict__ tables) {
   const uint32_t nLive = *a.dLive;
   const uint32_t nBlocks = (nLive + BS - 1u) / BS;
@@ -1078,7 +1091,8 @@ __global__ __launch_bounds__(BS) void k_candidates(CandArgs a, const uint32_t* _
   const float4 p = active ? a.sortedPT[j] : make_float4(0.f, 0.f, 0.f, 0.f);
   // replica of the wave's first key, then lanes move forward past later
   // replica starts (keys ascend through the wave)
-  const int rw = replica_of_bucket(bucketStart, a.R, __builtin_amdgcn_readfirstlane(key));
+  const uint32_t key0 = __builtin_amdgcn_readfirstlane(key);
+  const int rw = bucketRep ? bucketRep[key0 >> a.repShift] : replica_of_bucket(bucketStart, a.R, key0);
   int r = rw;
   if (active)
     while (r + 1 < a.R && bucketStart[r + 1] <= key) ++r;
@@ -1758,13 +1772,14 @@ hipError_t launch_candidates(const CandArgs& a, uint32_t maxN, int blockSize, hi
   // one result per sorted particle whatever the shape: waves are always 64
   // consecutive particles
   if (blockSize == 64)
-    hipLaunchKernelGGL(k_candidates<64>, dim3(cdiv(maxN, 64)), dim3(64), 0, stream, a, a.bucketStart, a.tables);
+    hipLaunchKernelGGL(k_candidates<64>, dim3(cdiv(maxN, 64)), dim3(64), 0, stream, a, a.bucketStart, a.bucketRep,
+                       a.tables);
   else if (blockSize == 128)
     hipLaunchKernelGGL(k_candidates<128>, dim3(cdiv(maxN, 128)), dim3(128), 0, stream, a, a.bucketStart,
-                       a.tables);
+                       a.bucketRep, a.tables);
   else
     hipLaunchKernelGGL(k_candidates<256>, dim3(cdiv(maxN, 256)), dim3(256), 0, stream, a, a.bucketStart,
-                       a.tables);
+                       a.bucketRep, a.tables);
   return hipGetLastError();
 }
 

@@ -56,6 +56,8 @@ class Engine {
   ReplicaTables* dTables = nullptr;
   int* dStreamReplica = nullptr;
   uint32_t* dBucketStart = nullptr;
+  int* dBucketRep = nullptr;  // replica of each 2^repShift-bucket chunk (nullptr: search bucketStart)
+  uint32_t repShift = 0;
   uint32_t *dRepOff = nullptr, *dRepCount = nullptr, *dNewOff = nullptr, *dNewCount = nullptr;
   uint32_t *dAliveBase = nullptr, *dBirthBase = nullptr, *dRepAlive = nullptr;
   uint32_t *dRepDeaths = nullptr, *dRepBirths = nullptr;

@@ -48,6 +48,8 @@ struct SortBuffers {
 struct CandArgs {
   const ReplicaTables* tables;
   const uint32_t* bucketStart;  // per replica bucketOff, [R] = deadKey
+  const int* bucketRep;         // replica of bucket chunk key >> repShift (nullptr: search bucketStart)
+  uint32_t repShift;
   int R;
   const uint32_t* keys;         // sorted
   const float4* sortedPT;
