@@ -20,7 +20,7 @@ MT_PARAM_SETS = 32768
 GAUSS_DIM = 4096
 
 BC_CLAMP, BC_REFLECT, BC_ABSORB, BC_PERIODIC = 0, 1, 2, 3
-KNOB_PRODUCER_BLOCK, KNOB_PAIR_BLOCK = 0, 1
+KNOB_PRODUCER_BLOCK, KNOB_PAIR_BLOCK, KNOB_SEGMENTED_SORT = 0, 1, 2
 KNOB_VALUES = {KNOB_PRODUCER_BLOCK: (128, 256, 512), KNOB_PAIR_BLOCK: (64, 128, 256)}
 
 
@@ -300,8 +300,9 @@ class Engine:
         _check(lib().smd_engine_set_sort(self._h, int(kind)), "smd_engine_set_sort")
 
     def set_knob(self, knob, value):
-        """Launch shape (KNOB_PRODUCER_BLOCK, KNOB_PAIR_BLOCK); results are
-        identical for every setting."""
+        """Launch shape (KNOB_PRODUCER_BLOCK, KNOB_PAIR_BLOCK) or sort layout
+        (KNOB_SEGMENTED_SORT: 1 sorts each replica on its local cells, 0 all
+        keys at once); results are identical for every setting."""
         _check(lib().smd_engine_set_knob(self._h, int(knob), int(value)), "smd_engine_set_knob")
 
     def get_knob(self, knob):

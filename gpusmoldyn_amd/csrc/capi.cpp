@@ -256,6 +256,7 @@ int smd_engine_get_knob(const smd_engine* e, int knob) {
   if (!e) return -SMD_ERR_INVALID;
   if (knob == SMD_KNOB_PRODUCER_BLOCK) return e->impl->producerBlock;
   if (knob == SMD_KNOB_PAIR_BLOCK) return e->impl->pairBlock;
+  if (knob == SMD_KNOB_SEGMENTED_SORT) return e->impl->segSort;
   return -SMD_ERR_INVALID;
 }
 

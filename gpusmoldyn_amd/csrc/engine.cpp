@@ -149,6 +149,7 @@ void Engine::free_particle_buffers() {
   dfree(dTileAlivePre);
   dfree(dTileBirthPre);
   dfree(dBlockSum);
+  dfree(dSegScratch);
 }
 
 int Engine::add_replica(const smd_sim_spec& spec, const float* pos4, const int* types, uint32_t n) {
@@ -198,6 +199,7 @@ smd_status Engine::alloc_particle_buffers(uint32_t c) {
   HIPCHK(dalloc(&dTileBirthPre, nTilesCap));
   blockStride = finalize_blocks(cap);
   HIPCHK(dalloc(&dBlockSum, 2 * (size_t)blockStride));
+  if (segBits) HIPCHK(dalloc(&dSegScratch, seg_sort_words(cap, (int)R, segBits)));
   HIPCHK(hipMemsetAsync(dTileAlive, 0, nTilesCap * sizeof(uint32_t), stream));
   HIPCHK(hipMemsetAsync(dTileBirth, 0, nTilesCap * sizeof(uint32_t), stream));
   return SMD_OK;
@@ -245,6 +247,18 @@ smd_status Engine::finalize(uint32_t capacity) {
   // workgroup and 2048 scan rows)
   if (radixBits == 0) radixBits = 8;
   nb = deadKey + 1;
+  // Ensembles sort each replica's segment on its local cells: two passes of
+  // `segBits` digits cover the largest replica grid plus the removed mark.
+  {
+    uint32_t maxGb = 0;
+    for (uint32_t r = 0; r < R; ++r) {
+      uint32_t b = 0;
+      while ((1u << b) < tabs[r].numBuckets) ++b;
+      maxGb = std::max(maxGb, b);
+    }
+    const uint32_t bits = std::max<uint32_t>(8u, (maxGb + 2u) / 2u);
+    segBits = (R > 1 && bits <= 10u) ? (int)bits : 0;
+  }
   Stot = (uint32_t)streams;
   uint64_t c = capacity ? capacity : std::max<uint64_t>(anySplit ? 8 * total : total, 1024);
   if (c < step_bound(total)) c = step_bound(total);
@@ -571,6 +585,24 @@ smd_status Engine::step_graph(int pairs) {
   return SMD_OK;
 }
 
+// The step's key sort: segmented by replica when the engine holds several
+// replicas and the knob allows it (the same sorted keys, values and records
+// either way), else over all keys at once.
+hipError_t Engine::sort_keys(uint32_t maxN) {
+  if (segBits && segSort && sb.algo == 1) {
+    SegSortArgs g;
+    g.R = (int)R;
+    g.segOff = dRepOff;
+    g.segCount = dRepCount;
+    g.bucketOff = dBucketStart;
+    g.deadKey = deadKey;
+    g.bits = segBits;
+    g.scratch = dSegScratch;
+    return launch_sort_seg(sb, g, dN, maxN, true, stream);
+  }
+  return launch_sort(sb, dN, maxN, keyBits, radixBits, true, stream);
+}
+
 smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
   auto mark = [&](int stage) {
     if (timer) (void)hipEventRecord(timer->ev[stage], stream);
@@ -615,7 +647,7 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
     sb.gatherDst = dSortedPT;
     sb.gatherDeadKey = deadKey;
     sb.sizeHint = (uint32_t)std::min<uint64_t>(knownN, 0xFFFFFFFFu);
-    HIPCHK(launch_sort(sb, dN, maxN, keyBits, radixBits, true, stream));
+    HIPCHK(sort_keys(maxN));
     mark(2);
     HIPCHK(launch_first(sb.keys, dN, maxN, nb, dFirst, dLive, dTables, slotsMode ? slotBits : 0u, R, stream));
     mark(3);
@@ -779,6 +811,10 @@ smd_status Engine::check_status() {
     set_error("sort look-back wait exceeded its bound");
     return SMD_ERR_HIP;
   }
+  if (v & kStatusSortLayout) {
+    set_error("segmented sort: a destination fell outside the particle array");
+    return SMD_ERR_HIP;
+  }
   return SMD_OK;
 }
 
@@ -791,6 +827,9 @@ smd_status Engine::set_knob(int knob, int value) {
   } else if (knob == SMD_KNOB_PAIR_BLOCK) {
     slot = &pairBlock;
     ok = value == 64 || value == 128 || value == 256;
+  } else if (knob == SMD_KNOB_SEGMENTED_SORT) {
+    slot = &segSort;
+    ok = value == 0 || value == 1;
   }
   if (!slot || !ok) {
     set_error("unknown knob or value out of range");
@@ -1105,7 +1144,7 @@ smd_status Engine::debug_stage(int stage, float* pos4, int* types, uint32_t* key
     sb.gatherDst = dSortedPT;
     sb.gatherDeadKey = deadKey;
     sb.sizeHint = (uint32_t)std::min<uint64_t>(knownN, 0xFFFFFFFFu);
-    HIPCHK(launch_sort(sb, dN, maxN, keyBits, radixBits, true, stream));
+    HIPCHK(sort_keys(maxN));
     HIPCHK(launch_first(sb.keys, dN, maxN, nb, dFirst, dLive, dTables, slotsMode ? slotBits : 0u, R, stream));
   }
   if (stage >= 3) {

@@ -1580,6 +1580,447 @@ __global__ __launch_bounds__(256) void k_mt_init_range(uint32_t* __restrict__ st
 }
 
 // ---------------------------------------------------------------------------
+// Segmented sort of an ensemble's keys.  The canonical particle order is
+// replica-major: replica r holds [segOff[r], segOff[r] + segCount[r]), and
+// every key there is bucketOff[r] + (a cell of r's grid, below 2^localBits)
+// or deadKey.  Sorting each segment on its local cell index (deadKey taken as
+// all ones) and moving every segment's dead particles behind all live ones
+// gives exactly the stable sort of all keys by global key: live particles
+// replica by replica in cell order, then the removed ones in index order.  The
+// local index needs ~16-19 bits where the global key needs 24, so two digit
+// passes do what took three.  Tiles never straddle two segments; each pass:
+// per-tile digit counts (k_seg_up), per digit an exclusive scan over each
+// segment's tiles (k_seg_rowscan), per segment an exclusive scan over digits
+// (k_seg_bases), then ranking and scatter (k_seg_down, the body of
+// k_sort_down with segment-relative digit starts).
+// ---------------------------------------------------------------------------
+constexpr uint32_t kSegDeadLocal = 0xFFFFFFFFu;
+
+__device__ __forceinline__ uint32_t seg_local(uint32_t key, uint32_t bucketOff, uint32_t deadKey) {
+  return key == deadKey ? kSegDeadLocal : key - bucketOff;
+}
+
+// Tile table: tile t = (first element, element count, segment, 1 for a
+// segment's first tile); one workgroup over all segments.  Tiles stay inside
+// the particle count and the table inside its capacity; a segment layout
+// that disagrees with either sets the sort-layout status bit.
+__global__ __launch_bounds__(1024) void k_seg_tiles(const uint32_t* __restrict__ segOff,
+                                                    const uint32_t* __restrict__ segCount, int R, uint32_t tileSize,
+                                                    uint32_t maxTiles, uint4* __restrict__ desc,
+                                                    uint32_t* __restrict__ nTiles, const uint32_t* __restrict__ dN,
+                                                    uint32_t* __restrict__ status) {
+  __shared__ uint32_t wsum[1][16];
+  const uint32_t nAll = *dN;
+  bool bad = false;
+  uint32_t carry = 0;
+  for (uint32_t b = 0; b < (uint32_t)R; b += 1024) {
+    const uint32_t r = b + threadIdx.x;
+    const uint32_t cnt = r < (uint32_t)R ? segCount[r] : 0u;
+    const uint32_t v[1] = {(cnt + tileSize - 1u) / tileSize};
+    uint32_t ex[1], tot[1];
+    block_exscan1024_k<1>(v, ex, tot, wsum);
+    if (r < (uint32_t)R) {
+      const uint32_t off = segOff[r];
+      for (uint32_t k = 0; k < v[0]; ++k) {
+        const uint32_t idx = carry + ex[0] + k;
+        const uint32_t start = off + k * tileSize;
+        uint32_t len = min(tileSize, cnt - k * tileSize);
+        if (start >= nAll || len > nAll - start) {
+          bad = true;
+          len = start < nAll ? nAll - start : 0u;
+        }
+        if (idx < maxTiles)
+          desc[idx] = make_uint4(start, len, r, k == 0 ? 1u : 0u);
+        else
+          bad = true;
+      }
+    }
+    carry += tot[0];
+  }
+  if (threadIdx.x == 0) *nTiles = min(carry, maxTiles);
+  if (bad && status) atomicOr(status, kStatusSortLayout);
+}
+
+template <int BITS, int ITEMS>
+__global__ __launch_bounds__(256) void k_seg_up(const uint32_t* __restrict__ keys, const uint4* __restrict__ desc,
+                                                const uint32_t* __restrict__ nTiles,
+                                                const uint32_t* __restrict__ bucketOff, uint32_t deadKey,
+                                                uint32_t* __restrict__ counts, uint32_t stride, int shift) {
+  constexpr int RADIX = 1 << BITS;
+  __shared__ uint32_t hist[RADIX];
+  const uint32_t T = *nTiles;
+  if (blockIdx.x >= T) return;
+  const uint32_t tile = xcd_tile(blockIdx.x, T);
+  const uint4 dsc = desc[tile];
+  const uint32_t bo = bucketOff[dsc.z];
+  uint32_t kk[ITEMS];
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    const uint32_t e = k * 256 + threadIdx.x;
+    kk[k] = e < dsc.y ? keys[dsc.x + e] : 0u;
+  }
+  for (int d = threadIdx.x; d < RADIX; d += 256) hist[d] = 0;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    const bool valid = k * 256 + threadIdx.x < dsc.y;
+    const uint32_t d = (seg_local(kk[k], bo, deadKey) >> shift) & (RADIX - 1);
+    const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+    const uint64_t vm = __ballot(valid);
+    if (__all(d == d0 || !valid)) {
+      if ((threadIdx.x & 63u) == 0u && vm) atomicAdd(&hist[d0], (uint32_t)__popcll(vm));
+    } else if (valid) {
+      atomicAdd(&hist[d], 1u);
+    }
+  }
+  __syncthreads();
+  for (int d = threadIdx.x; d < RADIX; d += 256) counts[(size_t)d * stride + tile] = hist[d];
+}
+
+// Inclusive scan of (value, segment-start flag) pairs over the workgroup's
+// 256 threads: a flag restarts the sum.  Returns the thread's inclusive pair
+// and, in *exV / *exF, the pair of everything before it.
+__device__ __forceinline__ void block_segscan256(uint32_t v, bool f, uint32_t& incV, bool& incF, uint32_t& exV,
+                                                 bool& exF, uint32_t* wv, uint32_t* wf) {
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  uint32_t x = v;
+  bool g = f;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t px = __shfl_up(x, o, 64);
+    const bool pg = __shfl_up((int)g, o, 64) != 0;
+    if (lane >= (uint32_t)o) {
+      x = g ? x : x + px;
+      g = g || pg;
+    }
+  }
+  if (lane == 63) {
+    wv[w] = x;
+    wf[w] = g ? 1u : 0u;
+  }
+  __syncthreads();
+  uint32_t inV = 0;
+  bool inF = false;
+  for (uint32_t k = 0; k < w; ++k) {
+    const bool kf = wf[k] != 0u;
+    inV = kf ? wv[k] : inV + wv[k];
+    inF = inF || kf;
+  }
+  incV = g ? x : inV + x;
+  incF = inF || g;
+  const uint32_t pv = __shfl_up(incV, 1, 64);
+  const bool pf = __shfl_up((int)incF, 1, 64) != 0;
+  exV = lane == 0 ? inV : pv;
+  exF = lane == 0 ? inF : pf;
+  __syncthreads();
+}
+
+// Per digit (one workgroup each): counts[d][t] becomes the exclusive prefix
+// over the earlier tiles of t's segment; the segment's total goes to
+// segTot[seg][d] at its last tile.
+template <int BITS, int ITEMS>
+__global__ __launch_bounds__(256) void k_seg_rowscan(uint32_t* __restrict__ counts, uint32_t stride,
+                                                     const uint4* __restrict__ desc,
+                                                     const uint32_t* __restrict__ nTiles,
+                                                     uint32_t* __restrict__ segTot) {
+  constexpr int RADIX = 1 << BITS;
+  constexpr int PER = 8;
+  __shared__ uint32_t wv[4], wf[4];
+  __shared__ uint32_t s_lastV, s_lastF;
+  const uint32_t T = *nTiles;
+  const uint32_t d = blockIdx.x;
+  uint32_t* row = counts + (size_t)d * stride;
+  uint32_t carry = 0;  // running sum of the segment open at the chunk start
+  for (uint32_t b = 0; b < T; b += 256 * PER) {
+    const uint32_t e0 = b + threadIdx.x * PER;
+    uint32_t v[PER], sg[PER];
+    bool h[PER + 1];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const bool ok = e0 + i < T;
+      v[i] = ok ? row[e0 + i] : 0u;
+      const uint4 ds = ok ? desc[e0 + i] : make_uint4(0u, 0u, 0u, 1u);
+      h[i] = ds.w != 0u;
+      sg[i] = ds.z;
+    }
+    h[PER] = e0 + PER < T ? desc[e0 + PER].w != 0u : true;
+    uint32_t aggV = 0;
+    bool aggF = false;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      if (h[i]) {
+        aggV = 0;
+        aggF = true;
+      }
+      aggV += v[i];
+    }
+    uint32_t incV, exV;
+    bool incF, exF;
+    block_segscan256(aggV, aggF, incV, incF, exV, exF, wv, wf);
+    uint32_t run = exF ? exV : carry + exV;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const uint32_t e = e0 + i;
+      if (h[i]) run = 0;
+      if (e < T) {
+        row[e] = run;
+        if (h[i + 1]) segTot[(size_t)sg[i] * RADIX + d] = run + v[i];  // the segment's last tile
+      }
+      run += v[i];
+    }
+    // the last thread's inclusive pair is the chunk's
+    if (threadIdx.x == 255) {
+      s_lastV = incV;
+      s_lastF = incF ? 1u : 0u;
+    }
+    __syncthreads();
+    carry = s_lastF ? s_lastV : carry + s_lastV;
+    __syncthreads();
+  }
+}
+
+// Per segment (one workgroup each): exclusive scan over digits of the
+// segment's digit totals; for the last pass also its live and removed counts
+// (removed particles, and only they, have the top digit there).
+template <int BITS>
+__global__ __launch_bounds__(256) void k_seg_bases(const uint32_t* __restrict__ segTot,
+                                                   const uint32_t* __restrict__ segCount,
+                                                   uint32_t* __restrict__ segExcl, uint32_t* __restrict__ repAlive,
+                                                   uint32_t* __restrict__ repDead, int last) {
+  constexpr int RADIX = 1 << BITS;
+  constexpr int DPT = (RADIX + 255) / 256;
+  __shared__ uint32_t wsum[4];
+  __shared__ uint32_t tot;
+  const uint32_t r = blockIdx.x;
+  const uint32_t cnt = segCount[r];
+  if (cnt == 0u) {
+    if (last && threadIdx.x == 0) {
+      repAlive[r] = 0;
+      repDead[r] = 0;
+    }
+    return;
+  }
+  const uint32_t* st = segTot + (size_t)r * RADIX;
+  uint32_t v[DPT], sum = 0;
+#pragma unroll
+  for (int q = 0; q < DPT; ++q) {
+    const uint32_t d = threadIdx.x * DPT + q;
+    v[q] = d < (uint32_t)RADIX ? st[d] : 0u;
+    sum += v[q];
+  }
+  uint32_t run = block_exscan256(sum, wsum, &tot);
+#pragma unroll
+  for (int q = 0; q < DPT; ++q) {
+    const uint32_t d = threadIdx.x * DPT + q;
+    if (d < (uint32_t)RADIX) segExcl[(size_t)r * RADIX + d] = run;
+    run += v[q];
+  }
+  if (last && threadIdx.x == 0) {
+    const uint32_t dead = st[RADIX - 1];
+    repDead[r] = dead;
+    repAlive[r] = cnt - dead;
+  }
+}
+
+// Last pass: where each segment's live and removed particles start (live ones
+// replica by replica from 0, removed ones after all live ones).
+__global__ __launch_bounds__(1024) void k_seg_repscan(const uint32_t* __restrict__ repAlive,
+                                                      const uint32_t* __restrict__ repDead, int R,
+                                                      uint32_t* __restrict__ liveStart,
+                                                      uint32_t* __restrict__ deadStart,
+                                                      uint32_t* __restrict__ nLive) {
+  __shared__ uint32_t wsum[2][16];
+  uint32_t cA = 0, cD = 0;
+  for (uint32_t b = 0; b < (uint32_t)R; b += 1024) {
+    const uint32_t r = b + threadIdx.x;
+    const uint32_t v[2] = {r < (uint32_t)R ? repAlive[r] : 0u, r < (uint32_t)R ? repDead[r] : 0u};
+    uint32_t ex[2], tot[2];
+    block_exscan1024_k<2>(v, ex, tot, wsum);
+    if (r < (uint32_t)R) {
+      liveStart[r] = cA + ex[0];
+      deadStart[r] = cD + ex[1];
+    }
+    cA += tot[0];
+    cD += tot[1];
+  }
+  if (threadIdx.x == 0) *nLive = cA;
+}
+
+template <int BITS, int ITEMS, bool GATHER, bool LAST>
+__global__ __launch_bounds__(256) void k_seg_down(const uint32_t* __restrict__ keysIn,
+                                                  const uint32_t* __restrict__ valsIn,
+                                                  uint32_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut,
+                                                  const uint32_t* __restrict__ counts, uint32_t stride,
+                                                  const uint4* __restrict__ desc,
+                                                  const uint32_t* __restrict__ nTiles,
+                                                  const uint32_t* __restrict__ bucketOff, uint32_t deadKey,
+                                                  const uint32_t* __restrict__ segOff,
+                                                  const uint32_t* __restrict__ segExcl,
+                                                  const uint32_t* __restrict__ liveStart,
+                                                  const uint32_t* __restrict__ deadStart,
+                                                  const uint32_t* __restrict__ repAlive,
+                                                  const uint32_t* __restrict__ nLive, int shift, int identityVals,
+                                                  const float4* __restrict__ gatherSrc,
+                                                  float4* __restrict__ gatherDst, const uint32_t* __restrict__ dN,
+                                                  uint32_t* __restrict__ status) {
+  constexpr int RADIX = 1 << BITS;
+  constexpr int DPT = (RADIX + 255) / 256;
+  constexpr uint32_t TILE = 256u * ITEMS;
+  // every destination is checked against the particle count: a layout that
+  // disagrees with it is reported (sticky status bit), never written
+  const uint32_t nAll = *dN;
+  bool bad = false;
+  __shared__ uint32_t s_base[RADIX];
+  __shared__ uint32_t s_wcnt[4][RADIX];
+  __shared__ uint32_t s_tstart[RADIX];
+  __shared__ uint32_t s_key[TILE];
+  __shared__ uint32_t s_val[TILE];
+  __shared__ uint32_t wsum[4];
+  __shared__ uint32_t tot;
+
+  const uint32_t T = *nTiles;
+  if (blockIdx.x >= T) return;
+  const uint32_t tile = xcd_tile(blockIdx.x, T);
+  const uint4 dsc = desc[tile];
+  const uint32_t seg = dsc.z, len = dsc.y;
+  const uint32_t bo = bucketOff[seg];
+  const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+  const uint32_t base = w * (64 * ITEMS);  // tile-local
+  uint32_t kk[ITEMS], vv[ITEMS], rk[ITEMS];
+#pragma unroll
+  for (int it = 0; it < ITEMS; ++it) {
+    const uint32_t e = base + it * 64 + lane;
+    const bool valid = e < len;
+    kk[it] = valid ? keysIn[dsc.x + e] : 0u;
+    vv[it] = identityVals ? dsc.x + e : (valid ? valsIn[dsc.x + e] : 0u);
+  }
+  // where each digit of this tile starts: the segment's digit start (live
+  // and removed particles apart in the last pass) + the earlier tiles of the
+  // segment
+  {
+    const uint32_t* ex = segExcl + (size_t)seg * RADIX;
+    uint32_t segBase, deadBase = 0;
+    if (LAST) {
+      segBase = liveStart[seg];
+      deadBase = *nLive + deadStart[seg] - repAlive[seg];
+    } else {
+      segBase = segOff[seg];
+    }
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+      const uint32_t d = t * DPT + q;
+      if (d < (uint32_t)RADIX) {
+        const uint32_t b0 = (LAST && d == (uint32_t)RADIX - 1u) ? deadBase : segBase;
+        s_base[d] = b0 + ex[d] + counts[(size_t)d * stride + tile];
+      }
+    }
+  }
+  for (uint32_t d = t; d < (uint32_t)RADIX; d += 256) {
+    s_wcnt[0][d] = 0;
+    s_wcnt[1][d] = 0;
+    s_wcnt[2][d] = 0;
+    s_wcnt[3][d] = 0;
+  }
+  __syncthreads();
+
+  const uint64_t lt = lanemask_lt();
+  uint32_t dg[ITEMS];
+#pragma unroll
+  for (int it = 0; it < ITEMS; ++it) {
+    const uint32_t e = base + it * 64 + lane;
+    const bool valid = e < len;
+    const uint32_t d = (seg_local(kk[it], bo, deadKey) >> shift) & (RADIX - 1);
+    dg[it] = d;
+    uint64_t mm = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < BITS; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bb = __ballot(bit);
+      mm &= bit ? bb : ~bb;
+    }
+    const uint32_t prev = s_wcnt[w][d];
+    rk[it] = prev + (uint32_t)__popcll(mm & lt);
+    if (valid && (mm & lt) == 0ull) s_wcnt[w][d] = prev + (uint32_t)__popcll(mm);
+  }
+  __syncthreads();
+  {
+    uint32_t c[DPT][4], tsum = 0;
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+      const uint32_t d = t * DPT + q;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) c[q][k] = (d < (uint32_t)RADIX) ? s_wcnt[k][d] : 0u;
+      tsum += c[q][0] + c[q][1] + c[q][2] + c[q][3];
+    }
+    uint32_t run = block_exscan256(tsum, wsum, &tot);
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+      const uint32_t d = t * DPT + q;
+      if (d < (uint32_t)RADIX) {
+        s_wcnt[0][d] = 0;
+        s_wcnt[1][d] = c[q][0];
+        s_wcnt[2][d] = c[q][0] + c[q][1];
+        s_wcnt[3][d] = c[q][0] + c[q][1] + c[q][2];
+        s_tstart[d] = run;
+      }
+      run += c[q][0] + c[q][1] + c[q][2] + c[q][3];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int it = 0; it < ITEMS; ++it) {
+    const uint32_t e = base + it * 64 + lane;
+    if (e < len) {
+      const uint32_t d = dg[it];
+      const uint32_t lp = s_tstart[d] + s_wcnt[w][d] + rk[it];
+      s_key[lp] = kk[it];
+      s_val[lp] = vv[it];
+    }
+  }
+  __syncthreads();
+  if (GATHER) {
+    uint32_t dk[ITEMS], dv[ITEMS], dd[ITEMS];
+    float4 rec[ITEMS];
+#pragma unroll
+    for (int q = 0; q < ITEMS; ++q) {
+      const uint32_t idx = t + q * 256u;
+      const bool ok = idx < len;
+      const uint32_t k = ok ? s_key[idx] : deadKey;
+      const uint32_t d = (seg_local(k, bo, deadKey) >> shift) & (RADIX - 1);
+      dk[q] = k;
+      dv[q] = ok ? s_val[idx] : 0u;
+      dd[q] = ok ? s_base[d] + (idx - s_tstart[d]) : 0u;
+      rec[q] = (ok && k < deadKey) ? gatherSrc[dv[q]] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int q = 0; q < ITEMS; ++q) {
+      if (t + q * 256u < len) {
+        if (dd[q] >= nAll) {
+          bad = true;
+          continue;
+        }
+        keysOut[dd[q]] = dk[q];
+        valsOut[dd[q]] = dv[q];
+        if (dk[q] < deadKey) gatherDst[dd[q]] = rec[q];
+      }
+    }
+  } else {
+    for (uint32_t idx = t; idx < len; idx += 256) {
+      const uint32_t k = s_key[idx];
+      const uint32_t d = (seg_local(k, bo, deadKey) >> shift) & (RADIX - 1);
+      const uint32_t dst = s_base[d] + (idx - s_tstart[d]);
+      if (dst >= nAll) {
+        bad = true;
+        continue;
+      }
+      keysOut[dst] = k;
+      valsOut[dst] = s_val[idx];
+    }
+  }
+  if (bad && status) atomicOr(status, kStatusSortLayout);
+}
+
+// ---------------------------------------------------------------------------
 // Launch wrappers.
 // ---------------------------------------------------------------------------
 static inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
@@ -1666,6 +2107,89 @@ static hipError_t launch_sort_onesweep(SortBuffers& b, const uint32_t* dN, uint3
     hipLaunchKernelGGL(k_gather, dim3(maxTiles * (kSortTile / 256)), dim3(256), 0, stream, b.keys, b.vals,
                        b.gatherSrc, dN, b.gatherDeadKey, b.gatherDst);
   return hipGetLastError();
+}
+
+// Segmented sort: scratch layout (words) and launch.
+static int seg_items(int bits) { return bits <= 9 ? 16 : 8; }
+
+static uint32_t seg_max_tiles(uint32_t maxN, int R, int bits) {
+  return cdiv(maxN, 256u * seg_items(bits)) + (uint32_t)R;
+}
+
+size_t seg_sort_words(uint32_t maxN, int R, int bits) {
+  const size_t radix = (size_t)1 << bits;
+  const size_t mt = seg_max_tiles(maxN, R, bits);
+  return 4 * mt + 64 + radix * (mt + 1) + 2 * (size_t)R * radix + 4 * (size_t)R + 64;
+}
+
+template <int BITS, int ITEMS>
+static hipError_t seg_sort_run(SortBuffers& b, const SegSortArgs& g, const uint32_t* dN, uint32_t maxN,
+                               bool identityVals, hipStream_t stream) {
+  constexpr int RADIX = 1 << BITS;
+  constexpr uint32_t TILE = 256u * ITEMS;
+  const uint32_t maxTiles = seg_max_tiles(maxN, g.R, BITS);
+  const uint32_t stride = maxTiles + 1;
+  uint32_t* w = g.scratch;
+  uint4* desc = (uint4*)w;
+  w += 4 * (size_t)maxTiles;
+  uint32_t* nTiles = w;
+  uint32_t* nLive = w + 1;
+  w += 64;
+  uint32_t* counts = w;
+  w += (size_t)RADIX * stride;
+  uint32_t* segTot = w;
+  w += (size_t)g.R * RADIX;
+  uint32_t* segExcl = w;
+  w += (size_t)g.R * RADIX;
+  uint32_t* repAlive = w;
+  uint32_t* repDead = w + g.R;
+  uint32_t* liveStart = w + 2 * (size_t)g.R;
+  uint32_t* deadStart = w + 3 * (size_t)g.R;
+  hipLaunchKernelGGL(k_seg_tiles, dim3(1), dim3(1024), 0, stream, g.segOff, g.segCount, g.R, TILE, maxTiles, desc,
+                     nTiles, dN, b.status);
+  for (int p = 0; p < 2; ++p) {
+    const int shift = p * BITS;
+    const bool last = p == 1;
+    hipLaunchKernelGGL((k_seg_up<BITS, ITEMS>), dim3(maxTiles), dim3(256), 0, stream, b.keys, desc, nTiles,
+                       g.bucketOff, g.deadKey, counts, stride, shift);
+    hipLaunchKernelGGL((k_seg_rowscan<BITS, ITEMS>), dim3(RADIX), dim3(256), 0, stream, counts, stride, desc, nTiles,
+                       segTot);
+    hipLaunchKernelGGL((k_seg_bases<BITS>), dim3(g.R), dim3(256), 0, stream, segTot, g.segCount, segExcl, repAlive,
+                       repDead, last ? 1 : 0);
+    const int idv = (identityVals && p == 0) ? 1 : 0;
+    if (!last) {
+      hipLaunchKernelGGL((k_seg_down<BITS, ITEMS, false, false>), dim3(maxTiles), dim3(256), 0, stream, b.keys,
+                         b.vals, b.keysAlt, b.valsAlt, counts, stride, desc, nTiles, g.bucketOff, g.deadKey, g.segOff,
+                         segExcl, liveStart, deadStart, repAlive, nLive, shift, idv, nullptr, nullptr, dN, b.status);
+    } else {
+      hipLaunchKernelGGL(k_seg_repscan, dim3(1), dim3(1024), 0, stream, repAlive, repDead, g.R, liveStart, deadStart,
+                         nLive);
+      if (b.gatherDst)
+        hipLaunchKernelGGL((k_seg_down<BITS, ITEMS, true, true>), dim3(maxTiles), dim3(256), 0, stream, b.keys,
+                           b.vals, b.keysAlt, b.valsAlt, counts, stride, desc, nTiles, g.bucketOff, g.deadKey,
+                           g.segOff, segExcl, liveStart, deadStart, repAlive, nLive, shift, idv, b.gatherSrc,
+                           b.gatherDst, dN, b.status);
+      else
+        hipLaunchKernelGGL((k_seg_down<BITS, ITEMS, false, true>), dim3(maxTiles), dim3(256), 0, stream, b.keys,
+                           b.vals, b.keysAlt, b.valsAlt, counts, stride, desc, nTiles, g.bucketOff, g.deadKey,
+                           g.segOff, segExcl, liveStart, deadStart, repAlive, nLive, shift, idv, nullptr, nullptr, dN,
+                           b.status);
+    }
+    std::swap(b.keys, b.keysAlt);
+    std::swap(b.vals, b.valsAlt);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_sort_seg(SortBuffers& b, const SegSortArgs& g, const uint32_t* dN, uint32_t maxN,
+                           bool identityVals, hipStream_t stream) {
+  if (maxN == 0 || g.R <= 0) return hipSuccess;
+  switch (g.bits) {
+    case 8: return seg_sort_run<8, 16>(b, g, dN, maxN, identityVals, stream);
+    case 9: return seg_sort_run<9, 16>(b, g, dN, maxN, identityVals, stream);
+    case 10: return seg_sort_run<10, 8>(b, g, dN, maxN, identityVals, stream);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 hipError_t launch_sort(SortBuffers& b, const uint32_t* dN, uint32_t maxN, uint32_t keyBits, int radixBits,

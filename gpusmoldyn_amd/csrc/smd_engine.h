@@ -111,8 +111,11 @@ class Engine {
   std::vector<uint8_t> slotBusy;     // admitted and not yet retired
   std::vector<uint8_t> slotPending;  // admitted, particles not yet entered
 
-  // launch shapes (identical results for every setting; see set_knob)
-  int producerBlock = 256, pairBlock = 256;
+  // launch shapes and the sort's form (identical results for every setting;
+  // see set_knob)
+  int producerBlock = 256, pairBlock = 256, segSort = 1;
+  int segBits = 0;                   // digit width of the segmented sort, 0: not applicable
+  uint32_t* dSegScratch = nullptr;   // its tile table and counters
   smd_status set_knob(int knob, int value);
 
   // hipGraph replay of two-step pairs, one executable graph per ping-pong
@@ -126,6 +129,7 @@ class Engine {
 
  private:
   smd_status enqueue_kernels(uint32_t maxN, StageTimer* timer);
+  hipError_t sort_keys(uint32_t maxN);
   smd_status step_graph(int pairs);
   smd_status alloc_particle_buffers(uint32_t c);
   void free_particle_buffers();

@@ -45,6 +45,19 @@ struct SortBuffers {
   uint32_t sizeHint;  // expected key count (tile size choice only), 0: maxN
 };
 
+// Segments of an ensemble's replica-major keys for the segmented sort:
+// segment r is [segOff[r], segOff[r] + segCount[r]), its keys are
+// bucketOff[r] + (a local cell below 2^(2 bits - 1)) or deadKey.
+struct SegSortArgs {
+  int R;
+  const uint32_t* segOff;
+  const uint32_t* segCount;
+  const uint32_t* bucketOff;
+  uint32_t deadKey;
+  int bits;            // digit width: two passes of 8, 9 or 10 bits
+  uint32_t* scratch;   // seg_sort_words(maxN, R, bits) words
+};
+
 struct CandArgs {
   const ReplicaTables* tables;
   const uint32_t* bucketStart;  // per replica bucketOff, [R] = deadKey
@@ -155,6 +168,12 @@ int sort_passes(uint32_t keyBits, int radixBits);
 size_t sort_counts_words(uint32_t maxN, int radixBits);
 hipError_t launch_sort(SortBuffers& b, const uint32_t* dN, uint32_t maxN, uint32_t keyBits, int radixBits,
                        bool identityVals, hipStream_t stream);
+// Segmented two-pass sort of an ensemble's keys (same output as launch_sort
+// with identityVals over all keys: live keys in order, removed ones after
+// them in index order).
+size_t seg_sort_words(uint32_t maxN, int R, int bits);
+hipError_t launch_sort_seg(SortBuffers& b, const SegSortArgs& g, const uint32_t* dN, uint32_t maxN,
+                           bool identityVals, hipStream_t stream);
 // Counter words the count / scan / scatter sort (SortBuffers::algo = 1)
 // uses for maxN keys: per-tile digit counts and the digit totals.
 size_t sort3k_counts_words(uint32_t maxN, uint32_t sizeHint, int radixBits);

@@ -28,6 +28,7 @@ enum Boundary : int32_t {
 // Sticky engine status bits, set by kernels and reported by the host.
 constexpr uint32_t kStatusSortSpin = 1u;      // a sort look-back wait hit its bound
 constexpr uint32_t kStatusZeroOverflow = 2u;  // order-0 creations exceeded the per-step limit
+constexpr uint32_t kStatusSortLayout = 4u;    // a segmented sort destination fell outside the particles
 
 struct ReplicaTables {
   int32_t numTypes;

@@ -185,7 +185,14 @@ smd_status smd_engine_set_graphs(smd_engine* e, int on);
  * 128, 256 = default; one thread per sorted particle).  No counterpart in
  * the reference (fixed 256-thread diffuse and 64-thread collide blocks,
  * particleSystemCuda.cu:167, 336). */
-typedef enum smd_knob { SMD_KNOB_PRODUCER_BLOCK = 0, SMD_KNOB_PAIR_BLOCK = 1 } smd_knob;
+typedef enum smd_knob {
+  SMD_KNOB_PRODUCER_BLOCK = 0,
+  SMD_KNOB_PAIR_BLOCK = 1,
+  /* 1 (default): an engine of several replicas sorts each replica's keys on
+   * its local cells (two digit passes where the global keys need three);
+   * 0: one sort over all keys */
+  SMD_KNOB_SEGMENTED_SORT = 2
+} smd_knob;
 smd_status smd_engine_set_knob(smd_engine* e, int knob, int value);
 /* Current value of a knob, or -SMD_ERR_INVALID. */
 int smd_engine_get_knob(const smd_engine* e, int knob);

@@ -410,6 +410,37 @@ def test_launch_shapes_agree(smd, gpu):
             e.close()
 
 
+@pytest.mark.parametrize("big", [(4, 4, 4), (6, 5, 5), (6, 6, 5), (6, 6, 6)])
+def test_segmented_sort_agrees(smd, gpu, big):
+    """An ensemble sorts replica by replica on local cells (digits of 8, 9 or
+    10 bits, from the largest grid: 2^12 to 2^18 buckets) or over all keys at
+    once: the same particles, bit for bit, with births, bimolecular deaths,
+    an absorbing face, an empty replica and one below a tile."""
+    reps = []
+    for k, (n, gb) in enumerate(((30000, big), (3000, (4, 4, 4)), (0, (3, 3, 3)), (200, (5, 4, 4)),
+                                  (12000, (5, 5, 5)), (45000, big))):
+        spec, pos, types = configs.c3(n=max(n, 1), n_blobs=4, sigma=1.0, n_streams=1024, grid_bits=gb, cell=0.15,
+                                      L=20.0, seed=300 + k)
+        spec.sphere[0][3] = 6.0
+        spec.rxn[0].rate = 2.0
+        reps.append((spec, pos[:n], types[:n]))
+    banner(f"segmented sort, largest grid {big}")
+    res = []
+    for seg in (1, 0):
+        e = smd.Engine(0)
+        for spec, pos, types in reps:
+            e.add_replica(spec, pos, types)
+        e.finalize()
+        e.set_knob(smd.KNOB_SEGMENTED_SORT, seg)
+        assert e.get_knob(smd.KNOB_SEGMENTED_SORT) == seg
+        e.step(8)
+        res.append([e.get(r) for r in range(len(reps))])
+        e.close()
+    for r, ((p1, t1), (p0, t0)) in enumerate(zip(*res)):
+        assert np.array_equal(t1, t0) and np.array_equal(p1.view(np.uint32), p0.view(np.uint32)), r
+    assert sum(len(t) for _, t in res[0]) > 0
+
+
 def test_tuned_ensemble_matches_oracle(smd, orc, gpu):
     """The per-size-class choices of the autotuner are valid simulations:
     replicas on refined and coarsened grids, with their own or fewer streams,
