@@ -199,7 +199,7 @@ smd_status Engine::alloc_particle_buffers(uint32_t c) {
   HIPCHK(dalloc(&dTileBirthPre, nTilesCap));
   blockStride = finalize_blocks(cap);
   HIPCHK(dalloc(&dBlockSum, 2 * (size_t)blockStride));
-  if (segBits) HIPCHK(dalloc(&dSegScratch, seg_sort_words(cap, (int)R, segBits)));
+  if (segHigh) HIPCHK(dalloc(&dSegScratch, seg_sort_words(cap, (int)R, segHigh)));
   HIPCHK(hipMemsetAsync(dTileAlive, 0, nTilesCap * sizeof(uint32_t), stream));
   HIPCHK(hipMemsetAsync(dTileBirth, 0, nTilesCap * sizeof(uint32_t), stream));
   return SMD_OK;
@@ -247,8 +247,8 @@ smd_status Engine::finalize(uint32_t capacity) {
   // workgroup and 2048 scan rows)
   if (radixBits == 0) radixBits = 8;
   nb = deadKey + 1;
-  // Ensembles sort each replica's segment on its local cells: two passes of
-  // `segBits` digits cover the largest replica grid plus the removed mark.
+  // Ensembles sort each replica's segment on its local cells: two digit
+  // passes cover the largest replica grid plus the removed mark.
   {
     uint32_t maxGb = 0;
     for (uint32_t r = 0; r < R; ++r) {
@@ -256,8 +256,8 @@ smd_status Engine::finalize(uint32_t capacity) {
       while ((1u << b) < tabs[r].numBuckets) ++b;
       maxGb = std::max(maxGb, b);
     }
-    const uint32_t bits = std::max<uint32_t>(8u, (maxGb + 2u) / 2u);
-    segBits = (R > 1 && bits <= 10u) ? (int)bits : 0;
+    seg_sort_digits(maxGb, segLow, segHigh);
+    if (R <= 1) segLow = segHigh = 0;
   }
   Stot = (uint32_t)streams;
   uint64_t c = capacity ? capacity : std::max<uint64_t>(anySplit ? 8 * total : total, 1024);
@@ -589,14 +589,15 @@ smd_status Engine::step_graph(int pairs) {
 // replicas and the knob allows it (the same sorted keys, values and records
 // either way), else over all keys at once.
 hipError_t Engine::sort_keys(uint32_t maxN) {
-  if (segBits && segSort && sb.algo == 1) {
+  if (segHigh && segSort && sb.algo == 1) {
     SegSortArgs g;
     g.R = (int)R;
     g.segOff = dRepOff;
     g.segCount = dRepCount;
     g.bucketOff = dBucketStart;
     g.deadKey = deadKey;
-    g.bits = segBits;
+    g.lowBits = segLow;
+    g.highBits = segHigh;
     g.scratch = dSegScratch;
     return launch_sort_seg(sb, g, dN, maxN, true, stream);
   }

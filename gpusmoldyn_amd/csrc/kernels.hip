@@ -1261,28 +1261,44 @@ __device__ __forceinline__ void block_exscan1024_k(const uint32_t (&v)[K], uint3
 }
 
 // Compaction tiles per finalize workgroup (1024 threads x 4 consecutive tiles).
-constexpr uint32_t kFinTPT = 4;
+constexpr uint32_t kFinTPT = 1;
 constexpr uint32_t kFinBlockTiles = 1024 * kFinTPT;
 
 // Per-step bookkeeping after the pair step.  Workgroup g scans the compaction
-// tiles [g * 4096, (g + 1) * 4096): exclusive prefixes of both counters inside
+// tiles [g * 1024, (g + 1) * 1024): exclusive prefixes of both counters inside
 // the block, the counters reset, the block's totals published for the
 // scatter (which adds the totals of the earlier blocks).  Workgroup 0 also
 // lays out every replica's new segment and publishes the step's counts.
 __global__ __launch_bounds__(1024) void k_finalize(FinalizeArgs a) {
   __shared__ uint32_t wsum[3][16];
-  const uint32_t n = *a.dN;  // read before any store (the arguments may alias)
-  const uint32_t nTiles = (n + kCompactTile - 1) / kCompactTile;
   const uint32_t t0 = blockIdx.x * kFinBlockTiles;
-  if (blockIdx.x > 0 && t0 >= nTiles) return;
-  // this block's compaction-tile counters are requested right away
-  uint32_t va[kFinTPT], vb[kFinTPT];
   const uint32_t e0 = t0 + threadIdx.x * kFinTPT;
+  // Every load that does not depend on the particle count is issued before
+  // that count arrives: this block's compaction-tile counters (up to the
+  // launch's tile bound; entries past the count are masked below) and the
+  // step counters.  All loads come before any store (the arguments may alias).
+  uint32_t va[kFinTPT], vb[kFinTPT];
 #pragma unroll
   for (uint32_t k = 0; k < kFinTPT; ++k) {
     const uint32_t e = e0 + k;
-    va[k] = e < nTiles ? a.tileAlive[e] : 0u;
-    vb[k] = e < nTiles ? a.tileBirth[e] : 0u;
+    va[k] = e < a.maxTiles ? a.tileAlive[e] : 0u;
+    vb[k] = e < a.maxTiles ? a.tileBirth[e] : 0u;
+  }
+  uint32_t stepPrev = 0;
+  unsigned long long psPrev = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    stepPrev = *a.stepCount;
+    psPrev = *a.particleSteps;
+  }
+  const uint32_t n = *a.dN;
+  const uint32_t nTiles = (n + kCompactTile - 1) / kCompactTile;
+  if (blockIdx.x > 0 && t0 >= nTiles) return;
+#pragma unroll
+  for (uint32_t k = 0; k < kFinTPT; ++k) {
+    if (e0 + k >= nTiles) {
+      va[k] = 0u;
+      vb[k] = 0u;
+    }
   }
   if (blockIdx.x == 0) {
     // replicas
@@ -1322,10 +1338,10 @@ __global__ __launch_bounds__(1024) void k_finalize(FinalizeArgs a) {
     if (threadIdx.x == 0) {
       a.newOff[a.R] = cN;
       *a.dNnext = cN;
-      *a.particleSteps += n;
+      *a.particleSteps = psPrev + n;
       // the host reads the particle count after each step from this ring,
       // tagged with the step number (no copy or event in the stream)
-      const uint32_t st = *a.stepCount + 1u;
+      const uint32_t st = stepPrev + 1u;
       *a.stepCount = st;
       __hip_atomic_store(&a.countRing[st & a.ringMask], ((unsigned long long)st << 32) | cN, __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_SYSTEM);
@@ -1588,26 +1604,32 @@ __global__ __launch_bounds__(256) void k_mt_init_range(uint32_t* __restrict__ st
 // gives exactly the stable sort of all keys by global key: live particles
 // replica by replica in cell order, then the removed ones in index order.  The
 // local index needs ~16-19 bits where the global key needs 24, so two digit
-// passes do what took three.  Tiles never straddle two segments; each pass:
-// per-tile digit counts (k_seg_up), per digit an exclusive scan over each
-// segment's tiles (k_seg_rowscan), per segment an exclusive scan over digits
-// (k_seg_bases), then ranking and scatter (k_seg_down, the body of
-// k_sort_down with segment-relative digit starts).
+// passes do what took three.  Tiles never straddle two segments.  Each pass:
+// per-tile digit counts (k_seg_up; the first pass also counts each segment's
+// removed particles), per digit an exclusive scan over each segment's tiles
+// (k_seg_rowscan; in the first pass one extra workgroup lays out where each
+// segment's live and removed particles end up), then ranking and scatter
+// (k_seg_down, the body of k_sort_down; each workgroup turns its segment's
+// digit totals into digit starts itself).
 // ---------------------------------------------------------------------------
 constexpr uint32_t kSegDeadLocal = 0xFFFFFFFFu;
+constexpr uint32_t kSegHead = 0x80000000u;  // tile table: first tile of its segment
 
 __device__ __forceinline__ uint32_t seg_local(uint32_t key, uint32_t bucketOff, uint32_t deadKey) {
   return key == deadKey ? kSegDeadLocal : key - bucketOff;
 }
 
-// Tile table: tile t = (first element, element count, segment, 1 for a
-// segment's first tile); one workgroup over all segments.  Tiles stay inside
-// the particle count and the table inside its capacity; a segment layout
-// that disagrees with either sets the sort-layout status bit.
+// Tile table: tile t covers keys [tpos[t].x, tpos[t].x + tpos[t].y) of
+// segment tseg[t] & ~kSegHead (kSegHead set on a segment's first tile); one
+// workgroup over all segments, which also clears the per-segment removed
+// counts.  Tiles stay inside the particle count and the table inside its
+// capacity; a segment layout that disagrees with either sets the sort-layout
+// status bit.
 __global__ __launch_bounds__(1024) void k_seg_tiles(const uint32_t* __restrict__ segOff,
                                                     const uint32_t* __restrict__ segCount, int R, uint32_t tileSize,
-                                                    uint32_t maxTiles, uint4* __restrict__ desc,
-                                                    uint32_t* __restrict__ nTiles, const uint32_t* __restrict__ dN,
+                                                    uint32_t maxTiles, uint2* __restrict__ tpos,
+                                                    uint32_t* __restrict__ tseg, uint32_t* __restrict__ nTiles,
+                                                    uint32_t* __restrict__ repDead, const uint32_t* __restrict__ dN,
                                                     uint32_t* __restrict__ status) {
   __shared__ uint32_t wsum[1][16];
   const uint32_t nAll = *dN;
@@ -1620,6 +1642,7 @@ __global__ __launch_bounds__(1024) void k_seg_tiles(const uint32_t* __restrict__
     uint32_t ex[1], tot[1];
     block_exscan1024_k<1>(v, ex, tot, wsum);
     if (r < (uint32_t)R) {
+      repDead[r] = 0u;
       const uint32_t off = segOff[r];
       for (uint32_t k = 0; k < v[0]; ++k) {
         const uint32_t idx = carry + ex[0] + k;
@@ -1629,10 +1652,12 @@ __global__ __launch_bounds__(1024) void k_seg_tiles(const uint32_t* __restrict__
           bad = true;
           len = start < nAll ? nAll - start : 0u;
         }
-        if (idx < maxTiles)
-          desc[idx] = make_uint4(start, len, r, k == 0 ? 1u : 0u);
-        else
+        if (idx < maxTiles) {
+          tpos[idx] = make_uint2(start, len);
+          tseg[idx] = r | (k == 0 ? kSegHead : 0u);
+        } else {
           bad = true;
+        }
       }
     }
     carry += tot[0];
@@ -1641,29 +1666,35 @@ __global__ __launch_bounds__(1024) void k_seg_tiles(const uint32_t* __restrict__
   if (bad && status) atomicOr(status, kStatusSortLayout);
 }
 
-template <int BITS, int ITEMS>
-__global__ __launch_bounds__(256) void k_seg_up(const uint32_t* __restrict__ keys, const uint4* __restrict__ desc,
+template <int BITS, int ITEMS, bool COUNT_DEAD>
+__global__ __launch_bounds__(256) void k_seg_up(const uint32_t* __restrict__ keys, const uint2* __restrict__ tpos,
+                                                const uint32_t* __restrict__ tseg,
                                                 const uint32_t* __restrict__ nTiles,
                                                 const uint32_t* __restrict__ bucketOff, uint32_t deadKey,
-                                                uint32_t* __restrict__ counts, uint32_t stride, int shift) {
+                                                uint32_t* __restrict__ counts, uint32_t stride, int shift,
+                                                uint32_t* __restrict__ repDead) {
   constexpr int RADIX = 1 << BITS;
   __shared__ uint32_t hist[RADIX];
+  __shared__ uint32_t s_dead;
   const uint32_t T = *nTiles;
   if (blockIdx.x >= T) return;
   const uint32_t tile = xcd_tile(blockIdx.x, T);
-  const uint4 dsc = desc[tile];
-  const uint32_t bo = bucketOff[dsc.z];
+  const uint2 ps = tpos[tile];
+  const uint32_t seg = tseg[tile] & ~kSegHead;
+  const uint32_t bo = bucketOff[seg];
   uint32_t kk[ITEMS];
 #pragma unroll
   for (int k = 0; k < ITEMS; ++k) {
     const uint32_t e = k * 256 + threadIdx.x;
-    kk[k] = e < dsc.y ? keys[dsc.x + e] : 0u;
+    kk[k] = e < ps.y ? keys[ps.x + e] : 0u;
   }
   for (int d = threadIdx.x; d < RADIX; d += 256) hist[d] = 0;
+  if (COUNT_DEAD && threadIdx.x == 0) s_dead = 0;
   __syncthreads();
+  uint32_t dead = 0;
 #pragma unroll
   for (int k = 0; k < ITEMS; ++k) {
-    const bool valid = k * 256 + threadIdx.x < dsc.y;
+    const bool valid = k * 256 + threadIdx.x < ps.y;
     const uint32_t d = (seg_local(kk[k], bo, deadKey) >> shift) & (RADIX - 1);
     const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
     const uint64_t vm = __ballot(valid);
@@ -1672,9 +1703,12 @@ __global__ __launch_bounds__(256) void k_seg_up(const uint32_t* __restrict__ key
     } else if (valid) {
       atomicAdd(&hist[d], 1u);
     }
+    if (COUNT_DEAD) dead += (uint32_t)__popcll(__ballot(valid && kk[k] == deadKey));
   }
+  if (COUNT_DEAD && (threadIdx.x & 63u) == 0u && dead) atomicAdd(&s_dead, dead);
   __syncthreads();
   for (int d = threadIdx.x; d < RADIX; d += 256) counts[(size_t)d * stride + tile] = hist[d];
+  if (COUNT_DEAD && threadIdx.x == 0 && s_dead) atomicAdd(&repDead[seg], s_dead);
 }
 
 // Inclusive scan of (value, segment-start flag) pairs over the workgroup's
@@ -1715,20 +1749,55 @@ __device__ __forceinline__ void block_segscan256(uint32_t v, bool f, uint32_t& i
   __syncthreads();
 }
 
+// Where each segment's particles end up after the last pass: live ones
+// replica by replica from 0 (liveStart), removed ones after all live ones in
+// replica order (nLive + deadStart).  Run by one 256-thread workgroup.
+__device__ void seg_layout(const uint32_t* __restrict__ segCount, const uint32_t* __restrict__ repDead, int R,
+                           uint32_t* __restrict__ liveStart, uint32_t* __restrict__ deadStart,
+                           uint32_t* __restrict__ nLive, uint32_t* wsum, uint32_t* tot) {
+  uint32_t cA = 0, cD = 0;
+  for (uint32_t b = 0; b < (uint32_t)R; b += 256) {
+    const uint32_t r = b + threadIdx.x;
+    const uint32_t dead = r < (uint32_t)R ? repDead[r] : 0u;
+    const uint32_t alive = r < (uint32_t)R ? segCount[r] - dead : 0u;
+    const uint32_t exA = block_exscan256(alive, wsum, tot);
+    const uint32_t totA = *tot;
+    const uint32_t exD = block_exscan256(dead, wsum, tot);
+    const uint32_t totD = *tot;
+    if (r < (uint32_t)R) {
+      liveStart[r] = cA + exA;
+      deadStart[r] = cD + exD;
+    }
+    cA += totA;
+    cD += totD;
+    __syncthreads();  // *tot is rewritten by the next chunk
+  }
+  if (threadIdx.x == 0) *nLive = cA;
+}
+
 // Per digit (one workgroup each): counts[d][t] becomes the exclusive prefix
 // over the earlier tiles of t's segment; the segment's total goes to
-// segTot[seg][d] at its last tile.
+// segTot[seg][d] at its last tile.  With `layout`, workgroup RADIX lays out
+// the segments' live and removed particles instead.
 template <int BITS, int ITEMS>
 __global__ __launch_bounds__(256) void k_seg_rowscan(uint32_t* __restrict__ counts, uint32_t stride,
-                                                     const uint4* __restrict__ desc,
+                                                     const uint32_t* __restrict__ tseg,
                                                      const uint32_t* __restrict__ nTiles,
-                                                     uint32_t* __restrict__ segTot) {
+                                                     uint32_t* __restrict__ segTot, int layout,
+                                                     const uint32_t* __restrict__ segCount,
+                                                     const uint32_t* __restrict__ repDead, int R,
+                                                     uint32_t* __restrict__ liveStart,
+                                                     uint32_t* __restrict__ deadStart, uint32_t* __restrict__ nLive) {
   constexpr int RADIX = 1 << BITS;
   constexpr int PER = 8;
   __shared__ uint32_t wv[4], wf[4];
   __shared__ uint32_t s_lastV, s_lastF;
-  const uint32_t T = *nTiles;
   const uint32_t d = blockIdx.x;
+  if (d >= (uint32_t)RADIX) {
+    if (layout) seg_layout(segCount, repDead, R, liveStart, deadStart, nLive, wv, &s_lastV);
+    return;
+  }
+  const uint32_t T = *nTiles;
   uint32_t* row = counts + (size_t)d * stride;
   uint32_t carry = 0;  // running sum of the segment open at the chunk start
   for (uint32_t b = 0; b < T; b += 256 * PER) {
@@ -1739,11 +1808,11 @@ __global__ __launch_bounds__(256) void k_seg_rowscan(uint32_t* __restrict__ coun
     for (int i = 0; i < PER; ++i) {
       const bool ok = e0 + i < T;
       v[i] = ok ? row[e0 + i] : 0u;
-      const uint4 ds = ok ? desc[e0 + i] : make_uint4(0u, 0u, 0u, 1u);
-      h[i] = ds.w != 0u;
-      sg[i] = ds.z;
+      const uint32_t ts = ok ? tseg[e0 + i] : kSegHead;
+      h[i] = (ts & kSegHead) != 0u;
+      sg[i] = ts & ~kSegHead;
     }
-    h[PER] = e0 + PER < T ? desc[e0 + PER].w != 0u : true;
+    h[PER] = e0 + PER < T ? (tseg[e0 + PER] & kSegHead) != 0u : true;
     uint32_t aggV = 0;
     bool aggF = false;
 #pragma unroll
@@ -1779,86 +1848,19 @@ __global__ __launch_bounds__(256) void k_seg_rowscan(uint32_t* __restrict__ coun
   }
 }
 
-// Per segment (one workgroup each): exclusive scan over digits of the
-// segment's digit totals; for the last pass also its live and removed counts
-// (removed particles, and only they, have the top digit there).
-template <int BITS>
-__global__ __launch_bounds__(256) void k_seg_bases(const uint32_t* __restrict__ segTot,
-                                                   const uint32_t* __restrict__ segCount,
-                                                   uint32_t* __restrict__ segExcl, uint32_t* __restrict__ repAlive,
-                                                   uint32_t* __restrict__ repDead, int last) {
-  constexpr int RADIX = 1 << BITS;
-  constexpr int DPT = (RADIX + 255) / 256;
-  __shared__ uint32_t wsum[4];
-  __shared__ uint32_t tot;
-  const uint32_t r = blockIdx.x;
-  const uint32_t cnt = segCount[r];
-  if (cnt == 0u) {
-    if (last && threadIdx.x == 0) {
-      repAlive[r] = 0;
-      repDead[r] = 0;
-    }
-    return;
-  }
-  const uint32_t* st = segTot + (size_t)r * RADIX;
-  uint32_t v[DPT], sum = 0;
-#pragma unroll
-  for (int q = 0; q < DPT; ++q) {
-    const uint32_t d = threadIdx.x * DPT + q;
-    v[q] = d < (uint32_t)RADIX ? st[d] : 0u;
-    sum += v[q];
-  }
-  uint32_t run = block_exscan256(sum, wsum, &tot);
-#pragma unroll
-  for (int q = 0; q < DPT; ++q) {
-    const uint32_t d = threadIdx.x * DPT + q;
-    if (d < (uint32_t)RADIX) segExcl[(size_t)r * RADIX + d] = run;
-    run += v[q];
-  }
-  if (last && threadIdx.x == 0) {
-    const uint32_t dead = st[RADIX - 1];
-    repDead[r] = dead;
-    repAlive[r] = cnt - dead;
-  }
-}
-
-// Last pass: where each segment's live and removed particles start (live ones
-// replica by replica from 0, removed ones after all live ones).
-__global__ __launch_bounds__(1024) void k_seg_repscan(const uint32_t* __restrict__ repAlive,
-                                                      const uint32_t* __restrict__ repDead, int R,
-                                                      uint32_t* __restrict__ liveStart,
-                                                      uint32_t* __restrict__ deadStart,
-                                                      uint32_t* __restrict__ nLive) {
-  __shared__ uint32_t wsum[2][16];
-  uint32_t cA = 0, cD = 0;
-  for (uint32_t b = 0; b < (uint32_t)R; b += 1024) {
-    const uint32_t r = b + threadIdx.x;
-    const uint32_t v[2] = {r < (uint32_t)R ? repAlive[r] : 0u, r < (uint32_t)R ? repDead[r] : 0u};
-    uint32_t ex[2], tot[2];
-    block_exscan1024_k<2>(v, ex, tot, wsum);
-    if (r < (uint32_t)R) {
-      liveStart[r] = cA + ex[0];
-      deadStart[r] = cD + ex[1];
-    }
-    cA += tot[0];
-    cD += tot[1];
-  }
-  if (threadIdx.x == 0) *nLive = cA;
-}
-
 template <int BITS, int ITEMS, bool GATHER, bool LAST>
 __global__ __launch_bounds__(256) void k_seg_down(const uint32_t* __restrict__ keysIn,
                                                   const uint32_t* __restrict__ valsIn,
                                                   uint32_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut,
                                                   const uint32_t* __restrict__ counts, uint32_t stride,
-                                                  const uint4* __restrict__ desc,
+                                                  const uint2* __restrict__ tpos,
+                                                  const uint32_t* __restrict__ tseg,
                                                   const uint32_t* __restrict__ nTiles,
                                                   const uint32_t* __restrict__ bucketOff, uint32_t deadKey,
                                                   const uint32_t* __restrict__ segOff,
-                                                  const uint32_t* __restrict__ segExcl,
+                                                  const uint32_t* __restrict__ segTot,
                                                   const uint32_t* __restrict__ liveStart,
                                                   const uint32_t* __restrict__ deadStart,
-                                                  const uint32_t* __restrict__ repAlive,
                                                   const uint32_t* __restrict__ nLive, int shift, int identityVals,
                                                   const float4* __restrict__ gatherSrc,
                                                   float4* __restrict__ gatherDst, const uint32_t* __restrict__ dN,
@@ -1866,10 +1868,6 @@ __global__ __launch_bounds__(256) void k_seg_down(const uint32_t* __restrict__ k
   constexpr int RADIX = 1 << BITS;
   constexpr int DPT = (RADIX + 255) / 256;
   constexpr uint32_t TILE = 256u * ITEMS;
-  // every destination is checked against the particle count: a layout that
-  // disagrees with it is reported (sticky status bit), never written
-  const uint32_t nAll = *dN;
-  bool bad = false;
   __shared__ uint32_t s_base[RADIX];
   __shared__ uint32_t s_wcnt[4][RADIX];
   __shared__ uint32_t s_tstart[RADIX];
@@ -1881,9 +1879,13 @@ __global__ __launch_bounds__(256) void k_seg_down(const uint32_t* __restrict__ k
   const uint32_t T = *nTiles;
   if (blockIdx.x >= T) return;
   const uint32_t tile = xcd_tile(blockIdx.x, T);
-  const uint4 dsc = desc[tile];
-  const uint32_t seg = dsc.z, len = dsc.y;
+  const uint2 ps = tpos[tile];
+  const uint32_t seg = tseg[tile] & ~kSegHead, len = ps.y;
   const uint32_t bo = bucketOff[seg];
+  // every destination is checked against the particle count: a layout that
+  // disagrees with it is reported (sticky status bit), never written
+  const uint32_t nAll = *dN;
+  bool bad = false;
   const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
   const uint32_t base = w * (64 * ITEMS);  // tile-local
   uint32_t kk[ITEMS], vv[ITEMS], rk[ITEMS];
@@ -1891,28 +1893,38 @@ __global__ __launch_bounds__(256) void k_seg_down(const uint32_t* __restrict__ k
   for (int it = 0; it < ITEMS; ++it) {
     const uint32_t e = base + it * 64 + lane;
     const bool valid = e < len;
-    kk[it] = valid ? keysIn[dsc.x + e] : 0u;
-    vv[it] = identityVals ? dsc.x + e : (valid ? valsIn[dsc.x + e] : 0u);
+    kk[it] = valid ? keysIn[ps.x + e] : 0u;
+    vv[it] = identityVals ? ps.x + e : (valid ? valsIn[ps.x + e] : 0u);
   }
-  // where each digit of this tile starts: the segment's digit start (live
-  // and removed particles apart in the last pass) + the earlier tiles of the
-  // segment
+  // where each digit of this tile starts: the segment's digit start (its
+  // digit totals scanned here; live and removed particles apart in the last
+  // pass) + the earlier tiles of the segment
   {
-    const uint32_t* ex = segExcl + (size_t)seg * RADIX;
-    uint32_t segBase, deadBase = 0;
+    const uint32_t* st = segTot + (size_t)seg * RADIX;
+    uint32_t tv[DPT], sum = 0;
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+      const uint32_t d = t * DPT + q;
+      tv[q] = d < (uint32_t)RADIX ? st[d] : 0u;
+      sum += tv[q];
+    }
+    uint32_t run = block_exscan256(sum, wsum, &tot);
+    uint32_t liveBase = 0, deadBase = 0;
     if (LAST) {
-      segBase = liveStart[seg];
-      deadBase = *nLive + deadStart[seg] - repAlive[seg];
+      liveBase = liveStart[seg];
+      deadBase = *nLive + deadStart[seg];
     } else {
-      segBase = segOff[seg];
+      liveBase = segOff[seg];
     }
 #pragma unroll
     for (int q = 0; q < DPT; ++q) {
       const uint32_t d = t * DPT + q;
       if (d < (uint32_t)RADIX) {
-        const uint32_t b0 = (LAST && d == (uint32_t)RADIX - 1u) ? deadBase : segBase;
-        s_base[d] = b0 + ex[d] + counts[(size_t)d * stride + tile];
+        // removed particles, and only they, have the top digit in the last pass
+        const uint32_t b0 = (LAST && d == (uint32_t)RADIX - 1u) ? deadBase : liveBase + run;
+        s_base[d] = b0 + counts[(size_t)d * stride + tile];
       }
+      run += tv[q];
     }
   }
   for (uint32_t d = t; d < (uint32_t)RADIX; d += 256) {
@@ -2109,87 +2121,114 @@ static hipError_t launch_sort_onesweep(SortBuffers& b, const uint32_t* dN, uint3
   return hipGetLastError();
 }
 
-// Segmented sort: scratch layout (words) and launch.
-static int seg_items(int bits) { return bits <= 9 ? 16 : 8; }
+// Segmented sort: scratch layout (words) and launch.  Both passes share one
+// tile table (tiles of 4096 keys inside segments); the counters are sized
+// for the wider (second) digit.
+constexpr int kSegItems = 16;
 
-static uint32_t seg_max_tiles(uint32_t maxN, int R, int bits) {
-  return cdiv(maxN, 256u * seg_items(bits)) + (uint32_t)R;
+static uint32_t seg_max_tiles(uint32_t maxN, int R) { return cdiv(maxN, 256u * kSegItems) + (uint32_t)R; }
+
+size_t seg_sort_words(uint32_t maxN, int R, int highBits) {
+  const size_t radix = (size_t)1 << highBits;
+  const size_t mt = seg_max_tiles(maxN, R);
+  return 3 * mt + 64 + radix * (mt + 1) + (size_t)R * radix + 3 * (size_t)R + 64;
 }
 
-size_t seg_sort_words(uint32_t maxN, int R, int bits) {
-  const size_t radix = (size_t)1 << bits;
-  const size_t mt = seg_max_tiles(maxN, R, bits);
-  return 4 * mt + 64 + radix * (mt + 1) + 2 * (size_t)R * radix + 4 * (size_t)R + 64;
+void seg_sort_digits(uint32_t maxGridBits, int& lowBits, int& highBits) {
+  // the first pass keeps 8-bit digits where it can (runs of 16 keys per digit
+  // and tile keep its scattered stores in whole 64-byte pieces); removed
+  // particles need one value above every live cell
+  const uint32_t need = maxGridBits + 1u;
+  static const int opts[5][2] = {{8, 8}, {8, 9}, {8, 10}, {9, 10}, {10, 10}};
+  lowBits = highBits = 0;
+  for (const auto& o : opts)
+    if ((uint32_t)(o[0] + o[1]) >= need) {
+      lowBits = o[0];
+      highBits = o[1];
+      return;
+    }
 }
 
-template <int BITS, int ITEMS>
+namespace {
+struct SegScratch {
+  uint2* tpos;
+  uint32_t *tseg, *nTiles, *nLive, *counts, *segTot, *repDead, *liveStart, *deadStart;
+  uint32_t maxTiles, stride;
+};
+}  // namespace
+
+// One digit pass of the segmented sort (BITS wide at `shift`).  The first
+// pass also counts removed particles and lays out the segments; the last
+// places removed particles after every replica's live ones and gathers the
+// records when asked.
+template <int BITS, bool FIRST, bool LAST>
+static void seg_pass(SortBuffers& b, const SegSortArgs& g, const SegScratch& s, const uint32_t* dN, int shift,
+                     bool identityVals, hipStream_t stream) {
+  constexpr int RADIX = 1 << BITS;
+  const dim3 grid(s.maxTiles), blk(256);
+  hipLaunchKernelGGL((k_seg_up<BITS, kSegItems, FIRST>), grid, blk, 0, stream, b.keys, s.tpos, s.tseg, s.nTiles,
+                     g.bucketOff, g.deadKey, s.counts, s.stride, shift, s.repDead);
+  hipLaunchKernelGGL((k_seg_rowscan<BITS, kSegItems>), dim3(RADIX + (FIRST ? 1 : 0)), blk, 0, stream, s.counts,
+                     s.stride, s.tseg, s.nTiles, s.segTot, FIRST ? 1 : 0, g.segCount, s.repDead, g.R, s.liveStart,
+                     s.deadStart, s.nLive);
+  const int idv = identityVals ? 1 : 0;
+  if (!LAST)
+    hipLaunchKernelGGL((k_seg_down<BITS, kSegItems, false, false>), grid, blk, 0, stream, b.keys, b.vals, b.keysAlt,
+                       b.valsAlt, s.counts, s.stride, s.tpos, s.tseg, s.nTiles, g.bucketOff, g.deadKey, g.segOff,
+                       s.segTot, s.liveStart, s.deadStart, s.nLive, shift, idv, nullptr, nullptr, dN, b.status);
+  else if (b.gatherDst)
+    hipLaunchKernelGGL((k_seg_down<BITS, kSegItems, true, true>), grid, blk, 0, stream, b.keys, b.vals, b.keysAlt,
+                       b.valsAlt, s.counts, s.stride, s.tpos, s.tseg, s.nTiles, g.bucketOff, g.deadKey, g.segOff,
+                       s.segTot, s.liveStart, s.deadStart, s.nLive, shift, idv, b.gatherSrc, b.gatherDst, dN,
+                       b.status);
+  else
+    hipLaunchKernelGGL((k_seg_down<BITS, kSegItems, false, true>), grid, blk, 0, stream, b.keys, b.vals, b.keysAlt,
+                       b.valsAlt, s.counts, s.stride, s.tpos, s.tseg, s.nTiles, g.bucketOff, g.deadKey, g.segOff,
+                       s.segTot, s.liveStart, s.deadStart, s.nLive, shift, idv, nullptr, nullptr, dN, b.status);
+}
+
+template <int LB, int HB>
 static hipError_t seg_sort_run(SortBuffers& b, const SegSortArgs& g, const uint32_t* dN, uint32_t maxN,
                                bool identityVals, hipStream_t stream) {
-  constexpr int RADIX = 1 << BITS;
-  constexpr uint32_t TILE = 256u * ITEMS;
-  const uint32_t maxTiles = seg_max_tiles(maxN, g.R, BITS);
-  const uint32_t stride = maxTiles + 1;
+  SegScratch s;
+  s.maxTiles = seg_max_tiles(maxN, g.R);
+  s.stride = s.maxTiles + 1;
   uint32_t* w = g.scratch;
-  uint4* desc = (uint4*)w;
-  w += 4 * (size_t)maxTiles;
-  uint32_t* nTiles = w;
-  uint32_t* nLive = w + 1;
+  s.tpos = (uint2*)w;
+  w += 2 * (size_t)s.maxTiles;
+  s.tseg = w;
+  w += s.maxTiles;
+  s.nTiles = w;
+  s.nLive = w + 1;
   w += 64;
-  uint32_t* counts = w;
-  w += (size_t)RADIX * stride;
-  uint32_t* segTot = w;
-  w += (size_t)g.R * RADIX;
-  uint32_t* segExcl = w;
-  w += (size_t)g.R * RADIX;
-  uint32_t* repAlive = w;
-  uint32_t* repDead = w + g.R;
-  uint32_t* liveStart = w + 2 * (size_t)g.R;
-  uint32_t* deadStart = w + 3 * (size_t)g.R;
-  hipLaunchKernelGGL(k_seg_tiles, dim3(1), dim3(1024), 0, stream, g.segOff, g.segCount, g.R, TILE, maxTiles, desc,
-                     nTiles, dN, b.status);
-  for (int p = 0; p < 2; ++p) {
-    const int shift = p * BITS;
-    const bool last = p == 1;
-    hipLaunchKernelGGL((k_seg_up<BITS, ITEMS>), dim3(maxTiles), dim3(256), 0, stream, b.keys, desc, nTiles,
-                       g.bucketOff, g.deadKey, counts, stride, shift);
-    hipLaunchKernelGGL((k_seg_rowscan<BITS, ITEMS>), dim3(RADIX), dim3(256), 0, stream, counts, stride, desc, nTiles,
-                       segTot);
-    hipLaunchKernelGGL((k_seg_bases<BITS>), dim3(g.R), dim3(256), 0, stream, segTot, g.segCount, segExcl, repAlive,
-                       repDead, last ? 1 : 0);
-    const int idv = (identityVals && p == 0) ? 1 : 0;
-    if (!last) {
-      hipLaunchKernelGGL((k_seg_down<BITS, ITEMS, false, false>), dim3(maxTiles), dim3(256), 0, stream, b.keys,
-                         b.vals, b.keysAlt, b.valsAlt, counts, stride, desc, nTiles, g.bucketOff, g.deadKey, g.segOff,
-                         segExcl, liveStart, deadStart, repAlive, nLive, shift, idv, nullptr, nullptr, dN, b.status);
-    } else {
-      hipLaunchKernelGGL(k_seg_repscan, dim3(1), dim3(1024), 0, stream, repAlive, repDead, g.R, liveStart, deadStart,
-                         nLive);
-      if (b.gatherDst)
-        hipLaunchKernelGGL((k_seg_down<BITS, ITEMS, true, true>), dim3(maxTiles), dim3(256), 0, stream, b.keys,
-                           b.vals, b.keysAlt, b.valsAlt, counts, stride, desc, nTiles, g.bucketOff, g.deadKey,
-                           g.segOff, segExcl, liveStart, deadStart, repAlive, nLive, shift, idv, b.gatherSrc,
-                           b.gatherDst, dN, b.status);
-      else
-        hipLaunchKernelGGL((k_seg_down<BITS, ITEMS, false, true>), dim3(maxTiles), dim3(256), 0, stream, b.keys,
-                           b.vals, b.keysAlt, b.valsAlt, counts, stride, desc, nTiles, g.bucketOff, g.deadKey,
-                           g.segOff, segExcl, liveStart, deadStart, repAlive, nLive, shift, idv, nullptr, nullptr, dN,
-                           b.status);
-    }
-    std::swap(b.keys, b.keysAlt);
-    std::swap(b.vals, b.valsAlt);
-  }
+  s.counts = w;
+  w += ((size_t)1 << HB) * s.stride;
+  s.segTot = w;
+  w += (size_t)g.R << HB;
+  s.repDead = w;
+  s.liveStart = w + g.R;
+  s.deadStart = w + 2 * (size_t)g.R;
+  hipLaunchKernelGGL(k_seg_tiles, dim3(1), dim3(1024), 0, stream, g.segOff, g.segCount, g.R, 256u * kSegItems,
+                     s.maxTiles, s.tpos, s.tseg, s.nTiles, s.repDead, dN, b.status);
+  seg_pass<LB, true, false>(b, g, s, dN, 0, identityVals, stream);
+  std::swap(b.keys, b.keysAlt);
+  std::swap(b.vals, b.valsAlt);
+  seg_pass<HB, false, true>(b, g, s, dN, LB, false, stream);
+  std::swap(b.keys, b.keysAlt);
+  std::swap(b.vals, b.valsAlt);
   return hipGetLastError();
 }
 
 hipError_t launch_sort_seg(SortBuffers& b, const SegSortArgs& g, const uint32_t* dN, uint32_t maxN,
                            bool identityVals, hipStream_t stream) {
   if (maxN == 0 || g.R <= 0) return hipSuccess;
-  switch (g.bits) {
-    case 8: return seg_sort_run<8, 16>(b, g, dN, maxN, identityVals, stream);
-    case 9: return seg_sort_run<9, 16>(b, g, dN, maxN, identityVals, stream);
-    case 10: return seg_sort_run<10, 8>(b, g, dN, maxN, identityVals, stream);
-    default: return hipErrorInvalidValue;
-  }
+  const int lo = g.lowBits, hi = g.highBits;
+  if (lo == 8 && hi == 8) return seg_sort_run<8, 8>(b, g, dN, maxN, identityVals, stream);
+  if (lo == 8 && hi == 9) return seg_sort_run<8, 9>(b, g, dN, maxN, identityVals, stream);
+  if (lo == 8 && hi == 10) return seg_sort_run<8, 10>(b, g, dN, maxN, identityVals, stream);
+  if (lo == 9 && hi == 10) return seg_sort_run<9, 10>(b, g, dN, maxN, identityVals, stream);
+  if (lo == 10 && hi == 10) return seg_sort_run<10, 10>(b, g, dN, maxN, identityVals, stream);
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_sort(SortBuffers& b, const uint32_t* dN, uint32_t maxN, uint32_t keyBits, int radixBits,
@@ -2316,9 +2355,11 @@ uint32_t finalize_blocks(uint32_t maxN) {
   return std::max<uint32_t>(1u, cdiv(cdiv(maxN, kCompactTile), kFinBlockTiles));
 }
 
-hipError_t launch_finalize(const FinalizeArgs& a, uint32_t maxN, hipStream_t stream) {
+hipError_t launch_finalize(const FinalizeArgs& args, uint32_t maxN, hipStream_t stream) {
   const uint32_t g = finalize_blocks(maxN);
-  if (g > a.blockStride) return hipErrorInvalidValue;
+  if (g > args.blockStride) return hipErrorInvalidValue;
+  FinalizeArgs a = args;
+  a.maxTiles = cdiv(maxN, kCompactTile);
   hipLaunchKernelGGL(k_finalize, dim3(g), dim3(1024), 0, stream, a);
   return hipGetLastError();
 }

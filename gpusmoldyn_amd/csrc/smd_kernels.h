@@ -54,9 +54,15 @@ struct SegSortArgs {
   const uint32_t* segCount;
   const uint32_t* bucketOff;
   uint32_t deadKey;
-  int bits;            // digit width: two passes of 8, 9 or 10 bits
-  uint32_t* scratch;   // seg_sort_words(maxN, R, bits) words
+  int lowBits;         // first-pass digit width (8, 9 or 10)
+  int highBits;        // second-pass digit width (8, 9 or 10, at least lowBits)
+  uint32_t* scratch;   // seg_sort_words(maxN, R, highBits) words
 };
+
+// Digit widths of the segmented sort for replica grids of at most 2^maxGridBits
+// buckets: {low, high} with low + high > maxGridBits, or {0, 0} when two
+// passes of at most 10 bits cannot cover them.
+void seg_sort_digits(uint32_t maxGridBits, int& lowBits, int& highBits);
 
 struct CandArgs {
   const ReplicaTables* tables;
@@ -112,8 +118,9 @@ struct FinalizeArgs {
   uint32_t* injCount;      // per replica: particles admitted this step (nullptr: none); reset
   uint32_t* injBase;       // per replica: alive + births + created, where admitted particles go
   uint32_t* retire;        // per replica retire flags (nullptr: none); reset
-  uint32_t* blockSum;      // [2][blockStride]: alive and birth totals of each 4096-tile block
+  uint32_t* blockSum;      // [2][blockStride]: alive and birth totals of each 1024-tile block
   uint32_t blockStride;    // at least finalize_blocks(maxN)
+  uint32_t maxTiles;       // compaction tiles of the launch bound (set by launch_finalize)
 };
 
 struct ZeroArgs {
@@ -171,7 +178,7 @@ hipError_t launch_sort(SortBuffers& b, const uint32_t* dN, uint32_t maxN, uint32
 // Segmented two-pass sort of an ensemble's keys (same output as launch_sort
 // with identityVals over all keys: live keys in order, removed ones after
 // them in index order).
-size_t seg_sort_words(uint32_t maxN, int R, int bits);
+size_t seg_sort_words(uint32_t maxN, int R, int highBits);
 hipError_t launch_sort_seg(SortBuffers& b, const SegSortArgs& g, const uint32_t* dN, uint32_t maxN,
                            bool identityVals, hipStream_t stream);
 // Counter words the count / scan / scatter sort (SortBuffers::algo = 1)
