@@ -199,7 +199,7 @@ smd_status Engine::alloc_particle_buffers(uint32_t c) {
   HIPCHK(dalloc(&dTileBirthPre, nTilesCap));
   blockStride = finalize_blocks(cap);
   HIPCHK(dalloc(&dBlockSum, 2 * (size_t)blockStride));
-  if (segHigh) HIPCHK(dalloc(&dSegScratch, seg_sort_words(cap, (int)R, segHigh)));
+  if (segHigh) HIPCHK(dalloc(&dSegScratch, seg_sort_words(cap, (int)R, std::max(segLow, segHigh))));
   HIPCHK(hipMemsetAsync(dTileAlive, 0, nTilesCap * sizeof(uint32_t), stream));
   HIPCHK(hipMemsetAsync(dTileBirth, 0, nTilesCap * sizeof(uint32_t), stream));
   return SMD_OK;

@@ -2128,21 +2128,19 @@ constexpr int kSegItems = 16;
 
 static uint32_t seg_max_tiles(uint32_t maxN, int R) { return cdiv(maxN, 256u * kSegItems) + (uint32_t)R; }
 
-size_t seg_sort_words(uint32_t maxN, int R, int highBits) {
-  const size_t radix = (size_t)1 << highBits;
+size_t seg_sort_words(uint32_t maxN, int R, int maxBits) {
+  const size_t radix = (size_t)1 << maxBits;
   const size_t mt = seg_max_tiles(maxN, R);
   return 3 * mt + 64 + radix * (mt + 1) + (size_t)R * radix + 3 * (size_t)R + 64;
 }
 
 void seg_sort_digits(uint32_t maxGridBits, int& lowBits, int& highBits) {
-  // the first pass keeps 8-bit digits where it can (runs of 16 keys per digit
-  // and tile keep its scattered stores in whole 64-byte pieces); removed
-  // particles need one value above every live cell
+  // removed particles need one value above every live cell
   const uint32_t need = maxGridBits + 1u;
-  static const int opts[5][2] = {{8, 8}, {8, 9}, {8, 10}, {9, 10}, {10, 10}};
+  static const int opts[7][2] = {{8, 8}, {9, 8}, {9, 9}, {10, 9}, {10, 10}, {0, 0}, {0, 0}};
   lowBits = highBits = 0;
   for (const auto& o : opts)
-    if ((uint32_t)(o[0] + o[1]) >= need) {
+    if (o[0] && (uint32_t)(o[0] + o[1]) >= need) {
       lowBits = o[0];
       highBits = o[1];
       return;
@@ -2201,10 +2199,11 @@ static hipError_t seg_sort_run(SortBuffers& b, const SegSortArgs& g, const uint3
   s.nTiles = w;
   s.nLive = w + 1;
   w += 64;
+  constexpr int MB = LB > HB ? LB : HB;
   s.counts = w;
-  w += ((size_t)1 << HB) * s.stride;
+  w += ((size_t)1 << MB) * s.stride;
   s.segTot = w;
-  w += (size_t)g.R << HB;
+  w += (size_t)g.R << MB;
   s.repDead = w;
   s.liveStart = w + g.R;
   s.deadStart = w + 2 * (size_t)g.R;
@@ -2222,13 +2221,23 @@ static hipError_t seg_sort_run(SortBuffers& b, const SegSortArgs& g, const uint3
 hipError_t launch_sort_seg(SortBuffers& b, const SegSortArgs& g, const uint32_t* dN, uint32_t maxN,
                            bool identityVals, hipStream_t stream) {
   if (maxN == 0 || g.R <= 0) return hipSuccess;
-  const int lo = g.lowBits, hi = g.highBits;
-  if (lo == 8 && hi == 8) return seg_sort_run<8, 8>(b, g, dN, maxN, identityVals, stream);
-  if (lo == 8 && hi == 9) return seg_sort_run<8, 9>(b, g, dN, maxN, identityVals, stream);
-  if (lo == 8 && hi == 10) return seg_sort_run<8, 10>(b, g, dN, maxN, identityVals, stream);
-  if (lo == 9 && hi == 10) return seg_sort_run<9, 10>(b, g, dN, maxN, identityVals, stream);
-  if (lo == 10 && hi == 10) return seg_sort_run<10, 10>(b, g, dN, maxN, identityVals, stream);
-  return hipErrorInvalidValue;
+  // digit widths 7 to 10 bits in either pass
+#define SMD_SEG_HI(LO)                                                        \
+  switch (g.highBits) {                                                      \
+    case 7: return seg_sort_run<LO, 7>(b, g, dN, maxN, identityVals, stream);  \
+    case 8: return seg_sort_run<LO, 8>(b, g, dN, maxN, identityVals, stream);  \
+    case 9: return seg_sort_run<LO, 9>(b, g, dN, maxN, identityVals, stream);  \
+    case 10: return seg_sort_run<LO, 10>(b, g, dN, maxN, identityVals, stream); \
+    default: return hipErrorInvalidValue;                                    \
+  }
+  switch (g.lowBits) {
+    case 7: SMD_SEG_HI(7)
+    case 8: SMD_SEG_HI(8)
+    case 9: SMD_SEG_HI(9)
+    case 10: SMD_SEG_HI(10)
+    default: return hipErrorInvalidValue;
+  }
+#undef SMD_SEG_HI
 }
 
 hipError_t launch_sort(SortBuffers& b, const uint32_t* dN, uint32_t maxN, uint32_t keyBits, int radixBits,

@@ -54,9 +54,9 @@ struct SegSortArgs {
   const uint32_t* segCount;
   const uint32_t* bucketOff;
   uint32_t deadKey;
-  int lowBits;         // first-pass digit width (8, 9 or 10)
-  int highBits;        // second-pass digit width (8, 9 or 10, at least lowBits)
-  uint32_t* scratch;   // seg_sort_words(maxN, R, highBits) words
+  int lowBits;         // first-pass digit width (7 to 10 bits)
+  int highBits;        // second-pass digit width (7 to 10 bits)
+  uint32_t* scratch;   // seg_sort_words(maxN, R, max(lowBits, highBits)) words
 };
 
 // Digit widths of the segmented sort for replica grids of at most 2^maxGridBits
@@ -178,7 +178,7 @@ hipError_t launch_sort(SortBuffers& b, const uint32_t* dN, uint32_t maxN, uint32
 // Segmented two-pass sort of an ensemble's keys (same output as launch_sort
 // with identityVals over all keys: live keys in order, removed ones after
 // them in index order).
-size_t seg_sort_words(uint32_t maxN, int R, int highBits);
+size_t seg_sort_words(uint32_t maxN, int R, int maxBits);
 hipError_t launch_sort_seg(SortBuffers& b, const SegSortArgs& g, const uint32_t* dN, uint32_t maxN,
                            bool identityVals, hipStream_t stream);
 // Counter words the count / scan / scatter sort (SortBuffers::algo = 1)
