@@ -61,7 +61,8 @@ struct SegSortArgs {
 
 // Digit widths of the segmented sort for replica grids of at most 2^maxGridBits
 // buckets: {low, high} with low + high > maxGridBits, or {0, 0} when two
-// passes of at most 10 bits cannot cover them.
+// passes of at most 9 bits cannot cover them (the engine then sorts global
+// keys).
 void seg_sort_digits(uint32_t maxGridBits, int& lowBits, int& highBits);
 
 struct CandArgs {
