@@ -202,6 +202,7 @@ smd_status Engine::alloc_particle_buffers(uint32_t c) {
   if (segHigh) HIPCHK(dalloc(&dSegScratch, seg_sort_words(cap, (int)R, std::max(segLow, segHigh))));
   HIPCHK(hipMemsetAsync(dTileAlive, 0, nTilesCap * sizeof(uint32_t), stream));
   HIPCHK(hipMemsetAsync(dTileBirth, 0, nTilesCap * sizeof(uint32_t), stream));
+  HIPCHK(hipMemsetAsync(dBirthType, 0, (size_t)cap * sizeof(int), stream));
   return SMD_OK;
 }
 
@@ -727,6 +728,7 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
   sa.w = dW[cur];
   sa.addedPos = dAddedPos;
   sa.birthType = dBirthType;
+  sa.tileBirth = dTileBirth;
   sa.ptOut = dPT[cur ^ 1];
   sa.wOut = dW[cur ^ 1];
   sa.blockSum = dBlockSum;

@@ -148,7 +148,6 @@ __global__ __launch_bounds__(BS) void k_producer(ProducerArgs a, const ReplicaTa
       if (act && kill) {
         a.pt[i] = make_float4(pCur.x, pCur.y, pCur.z, __int_as_float(0));
         a.key[i] = a.deadKey;
-        a.birthType[i] = 0;
         db += 1u;
       } else if (act) {
         const float4 p = pCur;
@@ -214,8 +213,12 @@ __global__ __launch_bounds__(BS) void k_producer(ProducerArgs a, const ReplicaTa
         a.pt[i] = make_float4(np[0], np[1], np[2], __int_as_float(t));
         if (wZero) a.w[i] = 0.0f;
         a.key[i] = key;
-        a.birthType[i] = bt;
-        if (bt) a.addedPos[i] = make_float4(bp[0], bp[1], bp[2], bw);
+        // birth types are zero except where the scatter has a birth to take
+        // (it clears them again), so only births are written
+        if (bt) {
+          a.birthType[i] = bt;
+          a.addedPos[i] = make_float4(bp[0], bp[1], bp[2], bw);
+        }
         alive = t > 0;
         birth = bt > 0;
         db += (alive ? 0ull : 1ull) + (birth ? (1ull << 32) : 0ull);
@@ -1355,10 +1358,7 @@ __global__ __launch_bounds__(1024) void k_finalize(FinalizeArgs a) {
     const uint32_t e = e0 + k;
     sum[0] += va[k];
     sum[1] += vb[k];
-    if (e < nTiles) {
-      a.tileAlive[e] = 0;
-      a.tileBirth[e] = 0;
-    }
+    if (e < nTiles) a.tileAlive[e] = 0;  // the birth counters are cleared by the scatter
   }
   uint32_t ex[2], tot[2];
   block_exscan1024_k<2>(sum, ex, tot, wsum);
@@ -1401,13 +1401,19 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a, const uint32_t* 
   uint32_t ca = 0, cb = 0;
   // every load of the survivors' data before the first store (a wait for a
   // load also waits for the stores issued before it)
+  const uint32_t tileBirths = a.tileBirth[tile];  // births the producer counted here
 #pragma unroll
   for (int it = 0; it < kItems; ++it) {
     const uint32_t e = tile * kCompactTile + w * kWaveSpan + it * 64 + lane;
     const bool v = e < n;
     pv[it] = v ? a.pt[e] : make_float4(0.f, 0.f, 0.f, 0.f);
     wv[it] = v ? a.w[e] : 0.0f;
-    bt[it] = v ? a.birthType[e] : 0;
+  }
+  // birth types are read only in tiles with births (zero everywhere else)
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    const uint32_t e = tile * kCompactTile + w * kWaveSpan + it * 64 + lane;
+    bt[it] = (tileBirths != 0u && e < n) ? a.birthType[e] : 0;
   }
 #pragma unroll
   for (int it = 0; it < kItems; ++it) {
@@ -1482,8 +1488,10 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a, const uint32_t* 
       const float4 q = a.addedPos[e];
       a.ptOut[dst] = make_float4(q.x, q.y, q.z, __int_as_float(bt[it]));
       a.wOut[dst] = q.w;
+      a.birthType[e] = 0;  // taken: zero again for the next step
     }
   }
+  if (threadIdx.x == 0 && tileBirths != 0u) a.tileBirth[tile] = 0;
 }
 
 // ---------------------------------------------------------------------------

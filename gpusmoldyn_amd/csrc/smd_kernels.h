@@ -160,7 +160,8 @@ struct ScatterArgs {
   const float4* pt;
   const float* w;
   const float4* addedPos;
-  const int* birthType;
+  int* birthType;          // nonzero only at births; cleared as they are taken
+  uint32_t* tileBirth;     // births per compaction tile (producer); cleared here
   float4* ptOut;
   float* wOut;
   const uint32_t* blockSum;  // the finalize blocks' totals (FinalizeArgs::blockSum)
