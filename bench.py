@@ -52,8 +52,9 @@ def stage_bytes(name, n, S, nb, passes):
     """Algorithmic bytes of one launch of each stage for n particles, S RNG
     streams, nb buckets (compulsory reads + writes, each datum once)."""
     if name == "producer":
-        # record (x, y, z, type) 16 in; record 16, key 4, birth type 4 out; stream state
-        return 40 * n + 172 * S
+        # record (x, y, z, type) 16 in; record 16, key 4 out (birth type and position only at
+        # births); stream state
+        return 36 * n + 172 * S
     if name == "sort":
         # per digit pass: count (key 4) + scatter (key, value 8 in, 8 out); first pass values implicit;
         # the last pass also copies the packed record of each particle into sorted order (16 + 16)
@@ -65,7 +66,8 @@ def stage_bytes(name, n, S, nb, passes):
     if name == "resolve":
         return 4 * n
     if name == "scatter":
-        return 44 * n
+        # record 16 + fourth component 4 in, the same out (birth types only in tiles with births)
+        return 40 * n
     return 0
 
 
@@ -157,8 +159,7 @@ def roofline(e, cfg, nprof=5, streams=None, buckets=None):
     n = int(e.counts().sum())
     S = streams if streams is not None else lay["total_streams"]
     nb = buckets if buckets is not None else lay["total_buckets"]
-    rb = e.radix_bits()
-    passes = (lay["key_bits"] + rb - 1) // rb
+    passes = e.sort_plan()[0]  # two for an ensemble sorted replica by replica
     rows = {}
     for k, us in prof.items():
         b = stage_bytes(k, n, S, nb, passes)

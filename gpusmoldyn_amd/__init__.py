@@ -114,6 +114,7 @@ def lib():
         L.smd_engine_debug_stage.argtypes = [vp, ctypes.c_int, f32p, i32p, u32p, u32p, i32p,
                                              ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(ctypes.c_uint)]
         L.smd_engine_layout.argtypes = [vp] + [ctypes.POINTER(ctypes.c_uint)] * 4
+        L.smd_engine_sort_plan.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.smd_engine_profile_step.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
                                               ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.smd_mt_generate.argtypes = [ctypes.c_int, u32p, ctypes.c_uint, ctypes.c_int, ctypes.c_int, u32p]
@@ -349,6 +350,13 @@ class Engine:
 
     def radix_bits(self):
         return lib().smd_engine_radix_bits(self._h)
+
+    def sort_plan(self):
+        """(digit passes of the step's key sort, True when it sorts each
+        replica on its local cells)."""
+        p, g = ctypes.c_int(0), ctypes.c_int(0)
+        _check(lib().smd_engine_sort_plan(self._h, ctypes.byref(p), ctypes.byref(g)), "smd_engine_sort_plan")
+        return p.value, bool(g.value)
 
     def layout(self):
         v = [ctypes.c_uint(0) for _ in range(4)]

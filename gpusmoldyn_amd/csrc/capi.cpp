@@ -341,6 +341,14 @@ smd_status smd_engine_layout(smd_engine* e, uint32_t* total_buckets, uint32_t* k
   return SMD_OK;
 }
 
+smd_status smd_engine_sort_plan(const smd_engine* e, int* passes, int* segmented) {
+  if (!e || !e->impl->finalized) return SMD_ERR_INVALID;
+  const bool seg = e->impl->seg_sorted();
+  if (segmented) *segmented = seg ? 1 : 0;
+  if (passes) *passes = seg ? 2 : smd::sort_passes(e->impl->keyBits, e->impl->radixBits);
+  return SMD_OK;
+}
+
 smd_status smd_engine_profile_step(smd_engine* e, int nsteps, double* us, int max_stages, char* names, int names_cap,
                                    int* n_stages) {
   if (!e || !us) return SMD_ERR_INVALID;

@@ -589,8 +589,10 @@ smd_status Engine::step_graph(int pairs) {
 // The step's key sort: segmented by replica when the engine holds several
 // replicas and the knob allows it (the same sorted keys, values and records
 // either way), else over all keys at once.
+bool Engine::seg_sorted() const { return segHigh && segSort && sb.algo == 1; }
+
 hipError_t Engine::sort_keys(uint32_t maxN) {
-  if (segHigh && segSort && sb.algo == 1) {
+  if (seg_sorted()) {
     SegSortArgs g;
     g.R = (int)R;
     g.segOff = dRepOff;

@@ -127,6 +127,8 @@ class Engine {
   // `which` (-1: both).
   void drop_graph(int which = -1);
 
+  bool seg_sorted() const;  // this engine's step sorts replica by replica
+
  private:
   smd_status enqueue_kernels(uint32_t maxN, StageTimer* timer);
   hipError_t sort_keys(uint32_t maxN);

@@ -237,6 +237,10 @@ smd_status smd_engine_debug_first(smd_engine* e, uint32_t* first, uint32_t cap, 
 /* Number of buckets (keys deadKey = total buckets) and key bits. */
 smd_status smd_engine_layout(smd_engine* e, uint32_t* total_buckets, uint32_t* key_bits, uint32_t* total_streams,
                              uint32_t* capacity);
+/* How the step sorts its keys: *passes digit passes, *segmented = 1 when each
+ * replica is sorted on its local cells (SMD_KNOB_SEGMENTED_SORT applies),
+ * 0 for one sort over the global keys.  No counterpart in the reference. */
+smd_status smd_engine_sort_plan(const smd_engine* e, int* passes, int* segmented);
 /* Per-kernel timing of the last timed step (HIP events, microseconds):
  * names is a '\n' separated list written into buf. */
 smd_status smd_engine_profile_step(smd_engine* e, int nsteps, double* us, int max_stages, char* names, int names_cap,

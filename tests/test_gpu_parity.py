@@ -434,6 +434,9 @@ def test_segmented_sort_agrees(smd, gpu, big):
         e.finalize()
         e.set_knob(smd.KNOB_SEGMENTED_SORT, seg)
         assert e.get_knob(smd.KNOB_SEGMENTED_SORT) == seg
+        passes, segmented = e.sort_plan()
+        assert segmented == (seg == 1 and sum(big) <= 17)
+        assert passes == (2 if segmented else -(-e.layout()["key_bits"] // 8))
         e.step(8)
         res.append([e.get(r) for r in range(len(reps))])
         e.close()
