@@ -91,6 +91,31 @@ def test_c4_all_replicas(smd, orc, gpu):
     e.close()
 
 
+def test_c4_segmented_equals_global(smd, gpu):
+    """The full c4 ensemble in the benchmark's tuned layout (16 particles per
+    stream) stepped with the segmented sort (two passes) and with one sort
+    over the 9.25M global keys (three passes of 4096-key tiles): the same
+    particles, bit for bit."""
+    from gpusmoldyn_amd import autotune
+
+    reps = autotune.apply_ensemble(configs.c4(), {c: (0, 16) for c in range(10, 17)})
+    steps = 4
+    res = []
+    for seg in (1, 0):
+        banner(f"c4 tuned layout, segmented sort {seg}, {steps} steps")
+        e = smd.Engine(0)
+        for spec, pos, types in reps:
+            e.add_replica(spec, pos, types)
+        e.finalize()
+        e.set_knob(smd.KNOB_SEGMENTED_SORT, seg)
+        assert e.sort_plan() == ((2, True) if seg else (3, False))
+        e.step(steps)
+        res.append([e.get(r) for r in range(len(reps))])
+        e.close()
+    for r, ((p1, t1), (p0, t0)) in enumerate(zip(*res)):
+        _same(p1, t1, p0, t0, f"c4 replica {r}")
+
+
 def test_c5_jobs_match_oracle(smd, orc, gpu):
     """Jobs of the serving stream itself (of each species count 2, 3 and 4,
     its first three below 1e7 particle-steps), served together through the
