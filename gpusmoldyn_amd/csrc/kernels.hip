@@ -1076,31 +1076,22 @@ __device__ __forceinline__ void wave_search(const ReplicaTables& T, const CandAr
 // The bucket offsets and replica tables arrive as read-only non-aliased
 // arguments, so the wave's replica (searched once, for its first key) and the
 // table fields load through the scalar cache.
-constexpr uint32_t kCandRun = 16;  // consecutive pair-search blocks per XCD run
-
 template <int BS>
 __global__ __launch_bounds__(BS) void k_candidates(CandArgs a, const uint32_t* __restrict__ bucketStart,
                                                    const int* __restrict__ bucketRep,
                                                    const ReplicaTables* __restrict__ tables) {
-  // Blocks are dealt to the XCDs round robin; each XCD takes runs of
-  // kCandRun consecutive blocks (neighbouring cells share its L2), runs
-  // interleaved over the XCDs so that any live count keeps them balanced.
-  // The mapping does not depend on the live count, so the particle loads
-  // (bounded by the launch's bound) go out together with the count's.
-  const uint32_t b = blockIdx.x, run = (b >> 3) / kCandRun;
-  const uint32_t tile = run * (8u * kCandRun) + (b & 7u) * kCandRun + (b >> 3) % kCandRun;
-  const uint32_t j = tile * BS + threadIdx.x;
+  const uint32_t nLive = *a.dLive;
+  const uint32_t nBlocks = (nLive + BS - 1u) / BS;
+  if (blockIdx.x >= nBlocks) return;
+  // neighbouring blocks on one XCD: the cells they both search share its L2
+  const uint32_t j = xcd_tile(blockIdx.x, nBlocks) * BS + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wbase = j - lane;
-  const bool inBound = j < a.maxN;
-  const uint32_t keyB = inBound ? a.keys[j] : 0xFFFFFFFFu;
-  const float4 pB = inBound ? a.sortedPT[j] : make_float4(0.f, 0.f, 0.f, 0.f);
-  const uint32_t nLive = *a.dLive;
   if (wbase >= nLive) return;  // the whole wave is past the live particles
   const bool active = j < nLive;
   // inactive lanes get a key no live particle has
-  const uint32_t key = active ? keyB : 0xFFFFFFFFu;
-  const float4 p = active ? pB : make_float4(0.f, 0.f, 0.f, 0.f);
+  const uint32_t key = active ? a.keys[j] : 0xFFFFFFFFu;
+  const float4 p = active ? a.sortedPT[j] : make_float4(0.f, 0.f, 0.f, 0.f);
   // replica of the wave's first key, then lanes move forward past later
   // replica starts (keys ascend through the wave)
   const uint32_t key0 = __builtin_amdgcn_readfirstlane(key);
@@ -2359,22 +2350,19 @@ hipError_t launch_gather(const uint32_t* keys, const uint32_t* vals, const float
   return hipGetLastError();
 }
 
-hipError_t launch_candidates(const CandArgs& args, uint32_t maxN, int blockSize, hipStream_t stream) {
+hipError_t launch_candidates(const CandArgs& a, uint32_t maxN, int blockSize, hipStream_t stream) {
   if (maxN == 0) return hipSuccess;
-  CandArgs a = args;
-  a.maxN = maxN;
   // one result per sorted particle whatever the shape: waves are always 64
-  // consecutive particles; the grid is a whole number of XCD runs (the
-  // block-to-tile mapping is a bijection on it)
-  const uint32_t runBlocks = 8u * kCandRun;
-  const uint32_t bs = blockSize == 64 ? 64u : blockSize == 128 ? 128u : 256u;
-  const dim3 grid(cdiv(cdiv(maxN, bs), runBlocks) * runBlocks);
-  if (bs == 64)
-    hipLaunchKernelGGL(k_candidates<64>, grid, dim3(64), 0, stream, a, a.bucketStart, a.bucketRep, a.tables);
-  else if (bs == 128)
-    hipLaunchKernelGGL(k_candidates<128>, grid, dim3(128), 0, stream, a, a.bucketStart, a.bucketRep, a.tables);
+  // consecutive particles
+  if (blockSize == 64)
+    hipLaunchKernelGGL(k_candidates<64>, dim3(cdiv(maxN, 64)), dim3(64), 0, stream, a, a.bucketStart, a.bucketRep,
+                       a.tables);
+  else if (blockSize == 128)
+    hipLaunchKernelGGL(k_candidates<128>, dim3(cdiv(maxN, 128)), dim3(128), 0, stream, a, a.bucketStart,
+                       a.bucketRep, a.tables);
   else
-    hipLaunchKernelGGL(k_candidates<256>, grid, dim3(256), 0, stream, a, a.bucketStart, a.bucketRep, a.tables);
+    hipLaunchKernelGGL(k_candidates<256>, dim3(cdiv(maxN, 256)), dim3(256), 0, stream, a, a.bucketStart,
+                       a.bucketRep, a.tables);
   return hipGetLastError();
 }
 

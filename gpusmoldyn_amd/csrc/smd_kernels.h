@@ -76,7 +76,6 @@ struct CandArgs {
   const uint32_t* first;
   const uint32_t* dLive;
   int* cand;
-  uint32_t maxN;                // bound on the live particles (set by launch_candidates)
 };
 
 struct ResolveArgs {
