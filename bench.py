@@ -1,6 +1,6 @@
 """Benchmark of the gpusmoldyn timestep on one MI355X.
 
-  python bench.py [--steps K] [--warmup W] [--config all|c1|c2|c3|c4|c5]
+  python bench.py [--steps K] [--warmup W] [--config all|c1|c2|c3|c4|c5|c5mix]
 
 A step is one full timestep (diffuse + unimolecular reactions + boundaries +
 hash + sort + cell table + pair search + bimolecular reactions + compaction)
@@ -22,7 +22,8 @@ The default (`--config all`) measures what the headline metric names
   serving          c5, the Poisson job stream through the slot engine: p50 and
                    p99 job latency and aggregate particle-steps/s;
   cpu_baseline     the CPU oracle on a bounded sample of c4.
-`--config cN` runs one workload alone (profiling runs use these).
+`--config cN` runs one workload alone (profiling runs use these; c5mix is the
+serving engine at the fixed resident mix its roofline is measured on).
 """
 import argparse
 import json
@@ -120,7 +121,8 @@ def timed(e, steps):
 SINGLE_KERNEL = {"producer": "k_producer", "first": "k_first", "candidates": "k_candidates", "resolve": "k_resolve",
                  "scatter": "k_scatter"}
 PROFILE = {"c2": "step_kernels_c2_1M.md", "c3": "step_kernels_c3_4M.md", "c4": "step_kernels_c4_9M.md",
-           "c5": "step_kernels_c5_serving.md"}
+           "c5": "step_kernels_c5_mix8.md"}
+SERVING_RESIDENT = 8  # jobs resident in the serving engine's measured mix
 
 
 def profiled_traffic(cfg, kernel):
@@ -388,14 +390,13 @@ def serve(args):
     return jobs, st
 
 
-def serving_roofline(jobs, resident=8, nprof=5):
-    """Stage times (HIP events) and the dominant kernel's roofline of the
-    serving engine with a typical resident mix: the stream's first `resident`
-    jobs admitted together into the serving layout, stepped past their entry
-    step, then `nprof` eager steps timed."""
+def serving_mix(jobs, resident=SERVING_RESIDENT, graphs=True):
+    """The serving engine (32 slots of 2^18 buckets) with a typical resident
+    mix: the stream's first `resident` jobs admitted together and stepped past
+    their entry step.  Returns (server, streams in use, buckets in use)."""
     from gpusmoldyn_amd import serving
 
-    srv = serving.Server(n_slots=32, slot_bucket_bits=18, slot_streams=32768, capacity=1 << 24)
+    srv = serving.Server(n_slots=32, slot_bucket_bits=18, slot_streams=32768, capacity=1 << 24, graphs=graphs)
     e = srv.e
     mix = jobs[:resident]
     for k, j in enumerate(mix):
@@ -403,10 +404,46 @@ def serving_roofline(jobs, resident=8, nprof=5):
     e.step(4)
     streams = sum(int(j.spec.num_streams) for j in mix)
     buckets = sum(1 << sum(int(j.spec.grid_bits[d]) for d in range(3)) for j in mix)
+    return srv, streams, buckets
+
+
+def serving_roofline(jobs, nprof=5):
+    """Stage times (HIP events) and the dominant kernel's roofline of the
+    serving engine at its typical resident mix, `nprof` eager steps timed."""
+    srv, streams, buckets = serving_mix(jobs)
+    e = srv.e
     rl, step = roofline(e, "c5", nprof, streams=streams, buckets=buckets)
     step["resident_particles"] = int(e.counts().sum())
     srv.close()
     return rl, step
+
+
+def c5mix_line(args):
+    """The serving engine at its typical resident mix (the default stream's
+    first jobs), K timed steps after W warm-up steps: the workload of the
+    serving roofline, run alone so that a profiler's byte counters see it."""
+    from gpusmoldyn_amd import serving
+
+    jobs = serving.c5_jobs()
+    srv, streams, buckets = serving_mix(jobs, graphs=not args.no_graphs)
+    e = srv.e
+    e.step(args.warmup)
+    steps = args.steps if args.steps is not None else 20
+    ps, secs = timed(e, steps)
+    rl, step = roofline(e, "c5", 5, streams=streams, buckets=buckets)
+    out = {
+        "metric": METRIC, "value": ps / secs, "unit": "particle-steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": secs * 1e3 / max(steps, 1), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"c5 mix: the serving engine (32 slots of 2^18 buckets) with the default stream's "
+                               f"first {SERVING_RESIDENT} jobs resident", "resident_particles": int(e.counts().sum())},
+        "roofline": rl, "step_roofline": step,
+    }
+    if args.copy_probe:
+        us = smd.copy_probe(args.copy_probe, reps=10)
+        out["copy_probe"] = {"bytes": args.copy_probe, "avg_us": us, "gbs": 2 * args.copy_probe / (us * 1e-6) / 1e9}
+    srv.close()
+    return out
 
 
 def serving_report(jobs, st):
@@ -496,6 +533,9 @@ def main():
         sys.exit(2)
     if args.config == "c5":
         print(json.dumps(c5_line(args)), flush=True)
+        return
+    if args.config == "c5mix":
+        print(json.dumps(c5mix_line(args)), flush=True)
         return
     probe = None
     if args.copy_probe:
