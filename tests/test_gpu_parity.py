@@ -445,6 +445,34 @@ def test_segmented_sort_agrees(smd, gpu, big):
     assert sum(len(t) for _, t in res[0]) > 0
 
 
+def test_segmented_sort_many_replicas(smd, gpu):
+    """More replicas than one workgroup's lanes (1100 small replicas, some
+    empty): the segmented sort's tile table and layout scans loop over chunks
+    of replicas; results equal the global sort's bit for bit."""
+    rng = np.random.default_rng(5)
+    reps = []
+    for k in range(1100):
+        n = int(rng.integers(0, 160)) if k % 37 else 0
+        spec, pos, types = configs.c1(nA=n // 2, nB=n - n // 2, kf=50.0, n_streams=64, seed=1000 + k,
+                                      rng_seed=2000 + k, grid_bits=(3, 3, 3))
+        reps.append((spec, pos, types))
+    banner("segmented sort, 1100 replicas")
+    res = []
+    for seg in (1, 0):
+        e = smd.Engine(0)
+        for spec, pos, types in reps:
+            e.add_replica(spec, pos, types)
+        e.finalize()
+        e.set_knob(smd.KNOB_SEGMENTED_SORT, seg)
+        assert e.sort_plan()[1] == (seg == 1)
+        e.step(6)
+        res.append([e.get(r) for r in range(len(reps))])
+        e.close()
+    for r, ((p1, t1), (p0, t0)) in enumerate(zip(*res)):
+        assert np.array_equal(t1, t0) and np.array_equal(p1.view(np.uint32), p0.view(np.uint32)), r
+    assert sum(int((t == 3).sum()) for _, t in res[0]) > 0  # some pairs reacted
+
+
 def test_tuned_ensemble_matches_oracle(smd, orc, gpu):
     """The per-size-class choices of the autotuner are valid simulations:
     replicas on refined and coarsened grids, with their own or fewer streams,
