@@ -90,11 +90,14 @@ __global__ __launch_bounds__(BS) void k_producer(ProducerArgs a, const ReplicaTa
   __shared__ uint32_t lds_mt[kMtNN * BS];
   __shared__ float lds_gauss[kGaussDim];
   __shared__ ProducerLds lds_tab[BS / 64];
-  for (int k = threadIdx.x; k < kGaussDim; k += BS) lds_gauss[k] = a.gauss[k];
-
   const uint32_t g = blockIdx.x * BS + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63u;
   const int r = (g < a.Stot) ? a.streamReplica[g] : -1;
+  // a workgroup whose streams belong to no replica (the idle stream ranges of
+  // a slot engine) leaves before it copies the Gaussian table
+  if (!__syncthreads_or(r >= 0)) return;
+  for (int k = threadIdx.x; k < kGaussDim; k += BS) lds_gauss[k] = a.gauss[k];
+
   const int rw = __builtin_amdgcn_readfirstlane(r);  // a wave never spans two replicas
   ProducerLds& L = lds_tab[threadIdx.x >> 6];
   if (rw >= 0) {
