@@ -1614,8 +1614,9 @@ __global__ __launch_bounds__(256) void k_mt_init_range(uint32_t* __restrict__ st
 // all ones) and moving every segment's dead particles behind all live ones
 // gives exactly the stable sort of all keys by global key: live particles
 // replica by replica in cell order, then the removed ones in index order.  The
-// local index needs ~16-19 bits where the global key needs 24, so two digit
-// passes do what took three.  Tiles never straddle two segments.  Each pass:
+// local index of grids up to 2^17 buckets (plus the removed mark) fits two
+// digits of at most 9 bits where the global key needs three passes (24 bits
+// for c4's 512 grids).  Tiles never straddle two segments.  Each pass:
 // per-tile digit counts (k_seg_up; the first pass also counts each segment's
 // removed particles), per digit an exclusive scan over each segment's tiles
 // (k_seg_rowscan; in the first pass one extra workgroup lays out where each
