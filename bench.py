@@ -91,7 +91,9 @@ def build(cfg):
         reps = configs.c4()
         steps = configs.C4_STEPS
         desc = dict(workload="c4: 512 replicas of c1's network (A+B->C, reflective cube at 2.048 per um^3), "
-                             "n log-uniform 2k-64k, kf log-uniform 1e-4..1e-2, one stream per 8 particles",
+                             "n log-uniform 2k-64k, kf log-uniform 1e-4..1e-2; grids and particles per random "
+                             "stream autotuned per replica size class (before tuning: about one cell and 1/8 "
+                             "stream per particle)",
                     replicas=len(reps), particles=int(sum(len(r[2]) for r in reps)), parallelism="one GPU")
     else:
         raise SystemExit(f"unknown config {cfg}")
