@@ -320,6 +320,38 @@ def test_growth_beyond_capacity(smd, orc, gpu):
     assert np.array_equal(gt, rt) and np.array_equal(gp.view(np.uint32), rp.view(np.uint32))
 
 
+def test_ensemble_growth_beyond_capacity(smd, orc, gpu):
+    """Three replicas whose populations double every few steps (A -> 2A) in
+    an engine of small capacity: growth reallocates the particle buffers and
+    the segmented sort's scratch mid-run (graphs on); every replica equals
+    the oracle."""
+    reps = []
+    for k, (n, gb) in enumerate(((400, (3, 3, 3)), (700, (4, 3, 3)), (250, (3, 4, 3)))):
+        spec = configs.make_spec(D=[0.0, 0.2], dt=1e-2,
+                                 reactions=[dict(order=1, reactants=[1], products=[1, 1], rate=40.0)],
+                                 box_lo=[0.0] * 3, box_hi=[10.0] * 3, faces=[BC_REFLECT] * 6, grid_bits=gb,
+                                 num_streams=128 + 64 * k, seed=500 + k)
+        reps.append((spec, configs.uniform_box(np.random.default_rng(30 + k), n, 0.0, 10.0), np.ones(n, np.int32)))
+    banner("ensemble growth")
+    e = smd.Engine(0)
+    for spec, pos, types in reps:
+        e.add_replica(spec, pos, types)
+    e.finalize(capacity=2048)
+    assert e.sort_plan()[1]
+    e.step(12)
+    g = smd.gauss_table()
+    total = 0
+    for r, (spec, pos, types) in enumerate(reps):
+        sim = orc.sim_for(smd, spec, pos, types, cap=1 << 20)
+        sim.step(g, 12)
+        gp, gt = e.get(r)
+        rp, rt = sim.state()
+        assert np.array_equal(gt, rt) and np.array_equal(gp.view(np.uint32), rp.view(np.uint32)), r
+        total += len(gt)
+    assert total > 2048
+    e.close()
+
+
 def test_extinction_and_empty(smd, gpu):
     spec = configs.make_spec(D=[0.0, 0.2], dt=1e-1, reactions=[dict(order=1, reactants=[1], products=[], rate=100.0)],
                              box_lo=[0.0] * 3, box_hi=[10.0] * 3, faces=[BC_REFLECT] * 6, grid_bits=(3, 3, 3),
