@@ -352,6 +352,28 @@ def test_ensemble_growth_beyond_capacity(smd, orc, gpu):
     e.close()
 
 
+def test_ensemble_zeroth_order(smd, orc, gpu):
+    """Order-0 creation in an ensemble (segmented sort): particles created on
+    the device in each replica join its segment at compaction; every replica
+    equals the oracle."""
+    reps = [w_zeroth(seed=4357 + 11 * k) for k in range(3)]
+    banner("ensemble with order-0 creation")
+    e = smd.Engine(0)
+    for spec, pos, types in reps:
+        e.add_replica(spec, pos, types)
+    e.finalize()
+    assert e.sort_plan()[1]
+    e.step(15)
+    g = smd.gauss_table()
+    for r, (spec, pos, types) in enumerate(reps):
+        sim = orc.sim_for(smd, spec, pos, types)
+        sim.step(g, 15)
+        gp, gt = e.get(r)
+        rp, rt = sim.state()
+        assert np.array_equal(gt, rt) and np.array_equal(gp.view(np.uint32), rp.view(np.uint32)), r
+    e.close()
+
+
 def test_extinction_and_empty(smd, gpu):
     spec = configs.make_spec(D=[0.0, 0.2], dt=1e-1, reactions=[dict(order=1, reactants=[1], products=[], rate=100.0)],
                              box_lo=[0.0] * 3, box_hi=[10.0] * 3, faces=[BC_REFLECT] * 6, grid_bits=(3, 3, 3),
