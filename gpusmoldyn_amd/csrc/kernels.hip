@@ -481,23 +481,18 @@ __global__ __launch_bounds__(256) void k_sort_down(const uint32_t* __restrict__ 
   }
   __syncthreads();
 
-  const uint64_t lt = lanemask_lt();
 #pragma unroll
   for (int it = 0; it < ITEMS; ++it) {
     const uint32_t e = base + it * 64 + lane;
     const bool valid = e < n;
     const uint32_t k = kk[it];
     const uint32_t d = (k >> shift) & (RADIX - 1);
-    uint64_t mm = __ballot(valid);
-#pragma unroll
-    for (int b = 0; b < BITS; ++b) {
-      const bool bit = (d >> b) & 1u;
-      const uint64_t bb = __ballot(bit);
-      mm &= bit ? bb : ~bb;
-    }
+    const uint64_t mm = wave_digit_peers<BITS>(d, valid);
+    // every lane of the digit writes the same new count (a lane past the
+    // end counts its valid peers only, which is the same number)
     const uint32_t prev = s_wcnt[w][d];
-    rk[it] = prev + (uint32_t)__popcll(mm & lt);
-    if (valid && (mm & lt) == 0ull) s_wcnt[w][d] = prev + (uint32_t)__popcll(mm);
+    rk[it] = prev + lanes_below(mm);
+    s_wcnt[w][d] = prev + (uint32_t)__popcll(mm);
   }
   __syncthreads();
   // Exclusive scan over waves per digit, and over digits for the tile (the
@@ -642,7 +637,6 @@ __global__ __launch_bounds__(256) void k_os_pass(const uint32_t* __restrict__ ke
   // local ranks (wave64 ballot match per 64-element chunk, element order)
   const uint32_t base = tile * kSortTile + w * (64 * kSortItems);
   uint32_t kk[kSortItems], vv[kSortItems], rk[kSortItems];
-  const uint64_t lt = lanemask_lt();
 #pragma unroll
   for (int it = 0; it < kSortItems; ++it) {
     const uint32_t e = base + it * 64 + lane;
@@ -651,16 +645,12 @@ __global__ __launch_bounds__(256) void k_os_pass(const uint32_t* __restrict__ ke
     kk[it] = k;
     vv[it] = identityVals ? e : (valid ? valsIn[e] : 0u);
     const uint32_t d = (k >> shift) & (RADIX - 1);
-    uint64_t mm = __ballot(valid);
-#pragma unroll
-    for (int b = 0; b < BITS; ++b) {
-      const bool bit = (d >> b) & 1u;
-      const uint64_t bb = __ballot(bit);
-      mm &= bit ? bb : ~bb;
-    }
+    const uint64_t mm = wave_digit_peers<BITS>(d, valid);
+    // every lane of the digit writes the same new count (a lane past the
+    // end counts its valid peers only, which is the same number)
     const uint32_t prev = s_wcnt[w][d];
-    rk[it] = prev + (uint32_t)__popcll(mm & lt);
-    if (valid && (mm & lt) == 0ull) s_wcnt[w][d] = prev + (uint32_t)__popcll(mm);
+    rk[it] = prev + lanes_below(mm);
+    s_wcnt[w][d] = prev + (uint32_t)__popcll(mm);
   }
   __syncthreads();
   // per-digit wave offsets, tile counts, look-back, global bases
@@ -1947,7 +1937,6 @@ __global__ __launch_bounds__(256) void k_seg_down(const uint32_t* __restrict__ k
   }
   __syncthreads();
 
-  const uint64_t lt = lanemask_lt();
   uint32_t dg[ITEMS];
 #pragma unroll
   for (int it = 0; it < ITEMS; ++it) {
@@ -1955,16 +1944,12 @@ __global__ __launch_bounds__(256) void k_seg_down(const uint32_t* __restrict__ k
     const bool valid = e < len;
     const uint32_t d = (seg_local(kk[it], bo, deadKey) >> shift) & (RADIX - 1);
     dg[it] = d;
-    uint64_t mm = __ballot(valid);
-#pragma unroll
-    for (int b = 0; b < BITS; ++b) {
-      const bool bit = (d >> b) & 1u;
-      const uint64_t bb = __ballot(bit);
-      mm &= bit ? bb : ~bb;
-    }
+    const uint64_t mm = wave_digit_peers<BITS>(d, valid);
+    // every lane of the digit writes the same new count (a lane past the
+    // end counts its valid peers only, which is the same number)
     const uint32_t prev = s_wcnt[w][d];
-    rk[it] = prev + (uint32_t)__popcll(mm & lt);
-    if (valid && (mm & lt) == 0ull) s_wcnt[w][d] = prev + (uint32_t)__popcll(mm);
+    rk[it] = prev + lanes_below(mm);
+    s_wcnt[w][d] = prev + (uint32_t)__popcll(mm);
   }
   __syncthreads();
   {

@@ -296,6 +296,31 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
   return (lane == 0) ? 0ull : ((~0ull) >> (64u - lane));
 }
 
+// Number of set bits of m at lanes below this one (v_mbcnt).
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// The valid lanes of the wave whose BITS-bit digit equals this lane's.  For
+// each digit bit b the lanes that agree with ours are ballot(bit b) when our
+// bit is set and its complement when it is clear, that is ~(ballot ^ s) with
+// s our bit spread over 32 bits; acc & ~(ballot ^ s) is one three-input
+// bitwise operation per mask half (truth table over (acc, ballot, s): true
+// at 100 and 111, 0x90).
+template <int BITS>
+__device__ __forceinline__ uint64_t wave_digit_peers(uint32_t d, bool valid) {
+  const uint64_t v = __ballot(valid);
+  uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+#pragma unroll
+  for (int b = 0; b < BITS; ++b) {
+    const uint32_t s = (uint32_t)((int32_t)(d << (31 - b)) >> 31);
+    const uint64_t bb = __ballot(s != 0u);
+    lo = __builtin_amdgcn_bitop3_b32(lo, (uint32_t)bb, s, 0x90);
+    hi = __builtin_amdgcn_bitop3_b32(hi, (uint32_t)(bb >> 32), s, 0x90);
+  }
+  return ((uint64_t)hi << 32) | lo;
+}
+
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
