@@ -1607,33 +1607,39 @@ __global__ __launch_bounds__(256) void k_mt_init_range(uint32_t* __restrict__ st
 // local index of grids up to 2^17 buckets (plus the removed mark) fits two
 // digits of at most 9 bits where the global key needs three passes (24 bits
 // for c4's 512 grids).  Tiles never straddle two segments.  Each pass:
-// per-tile digit counts (k_seg_up; the first pass also counts each segment's
-// removed particles), per digit an exclusive scan over each segment's tiles
-// (k_seg_rowscan; in the first pass one extra workgroup lays out where each
-// segment's live and removed particles end up), then ranking and scatter
-// (k_seg_down, the body of k_sort_down; each workgroup turns its segment's
-// digit totals into digit starts itself).
+// per-tile digit counts (k_seg_up, one row of digits per tile; the first pass
+// also counts each segment's removed particles), per segment and digit an
+// exclusive scan over the segment's tiles (k_seg_scan; in the first pass one
+// extra workgroup lays out where each segment's live and removed particles
+// end up), then ranking and scatter (k_seg_down, the body of k_sort_down; each
+// workgroup turns its segment's digit totals into digit starts itself).
 // ---------------------------------------------------------------------------
 constexpr uint32_t kSegDeadLocal = 0xFFFFFFFFu;
 constexpr uint32_t kSegHead = 0x80000000u;  // tile table: first tile of its segment
+constexpr uint32_t kSegScanLong = 64;  // segments up to this many tiles: one scan workgroup each
 
 __device__ __forceinline__ uint32_t seg_local(uint32_t key, uint32_t bucketOff, uint32_t deadKey) {
   return key == deadKey ? kSegDeadLocal : key - bucketOff;
 }
 
 // Tile table: tile t covers keys [tpos[t].x, tpos[t].x + tpos[t].y) of
-// segment tseg[t] & ~kSegHead (kSegHead set on a segment's first tile); one
-// workgroup over all segments, which also clears the per-segment removed
-// counts.  Tiles stay inside the particle count and the table inside its
+// segment tseg[t] & ~kSegHead (kSegHead set on a segment's first tile), and
+// segment r's tiles start at segFirst[r]; *nLong counts the segments of more
+// than kSegScanLong tiles.  One workgroup over all segments, which also clears
+// the per-segment removed counts.  Tiles stay inside the particle count and the table inside its
 // capacity; a segment layout that disagrees with either sets the sort-layout
 // status bit.
 __global__ __launch_bounds__(1024) void k_seg_tiles(const uint32_t* __restrict__ segOff,
                                                     const uint32_t* __restrict__ segCount, int R, uint32_t tileSize,
                                                     uint32_t maxTiles, uint2* __restrict__ tpos,
                                                     uint32_t* __restrict__ tseg, uint32_t* __restrict__ nTiles,
+                                                    uint32_t* __restrict__ segFirst, uint32_t* __restrict__ nLong,
                                                     uint32_t* __restrict__ repDead, const uint32_t* __restrict__ dN,
                                                     uint32_t* __restrict__ status) {
   __shared__ uint32_t wsum[1][16];
+  __shared__ uint32_t s_long;
+  if (threadIdx.x == 0) s_long = 0;
+  __syncthreads();
   const uint32_t nAll = *dN;
   bool bad = false;
   uint32_t carry = 0;
@@ -1645,6 +1651,8 @@ __global__ __launch_bounds__(1024) void k_seg_tiles(const uint32_t* __restrict__
     block_exscan1024_k<1>(v, ex, tot, wsum);
     if (r < (uint32_t)R) {
       repDead[r] = 0u;
+      segFirst[r] = carry + ex[0];
+      if (v[0] > kSegScanLong) atomicAdd(&s_long, 1u);
       const uint32_t off = segOff[r];
       for (uint32_t k = 0; k < v[0]; ++k) {
         const uint32_t idx = carry + ex[0] + k;
@@ -1664,7 +1672,11 @@ __global__ __launch_bounds__(1024) void k_seg_tiles(const uint32_t* __restrict__
     }
     carry += tot[0];
   }
-  if (threadIdx.x == 0) *nTiles = min(carry, maxTiles);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    *nTiles = min(carry, maxTiles);
+    *nLong = s_long;
+  }
   if (bad && status) atomicOr(status, kStatusSortLayout);
 }
 
@@ -1673,7 +1685,7 @@ __global__ __launch_bounds__(256) void k_seg_up(const uint32_t* __restrict__ key
                                                 const uint32_t* __restrict__ tseg,
                                                 const uint32_t* __restrict__ nTiles,
                                                 const uint32_t* __restrict__ bucketOff, uint32_t deadKey,
-                                                uint32_t* __restrict__ counts, uint32_t stride, int shift,
+                                                uint32_t* __restrict__ counts, int shift,
                                                 uint32_t* __restrict__ repDead) {
   constexpr int RADIX = 1 << BITS;
   __shared__ uint32_t hist[RADIX];
@@ -1709,46 +1721,8 @@ __global__ __launch_bounds__(256) void k_seg_up(const uint32_t* __restrict__ key
   }
   if (COUNT_DEAD && (threadIdx.x & 63u) == 0u && dead) atomicAdd(&s_dead, dead);
   __syncthreads();
-  for (int d = threadIdx.x; d < RADIX; d += 256) counts[(size_t)d * stride + tile] = hist[d];
+  for (int d = threadIdx.x; d < RADIX; d += 256) counts[(size_t)tile * RADIX + d] = hist[d];
   if (COUNT_DEAD && threadIdx.x == 0 && s_dead) atomicAdd(&repDead[seg], s_dead);
-}
-
-// Inclusive scan of (value, segment-start flag) pairs over the workgroup's
-// 256 threads: a flag restarts the sum.  Returns the thread's inclusive pair
-// and, in *exV / *exF, the pair of everything before it.
-__device__ __forceinline__ void block_segscan256(uint32_t v, bool f, uint32_t& incV, bool& incF, uint32_t& exV,
-                                                 bool& exF, uint32_t* wv, uint32_t* wf) {
-  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
-  uint32_t x = v;
-  bool g = f;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t px = __shfl_up(x, o, 64);
-    const bool pg = __shfl_up((int)g, o, 64) != 0;
-    if (lane >= (uint32_t)o) {
-      x = g ? x : x + px;
-      g = g || pg;
-    }
-  }
-  if (lane == 63) {
-    wv[w] = x;
-    wf[w] = g ? 1u : 0u;
-  }
-  __syncthreads();
-  uint32_t inV = 0;
-  bool inF = false;
-  for (uint32_t k = 0; k < w; ++k) {
-    const bool kf = wf[k] != 0u;
-    inV = kf ? wv[k] : inV + wv[k];
-    inF = inF || kf;
-  }
-  incV = g ? x : inV + x;
-  incF = inF || g;
-  const uint32_t pv = __shfl_up(incV, 1, 64);
-  const bool pf = __shfl_up((int)incF, 1, 64) != 0;
-  exV = lane == 0 ? inV : pv;
-  exF = lane == 0 ? inF : pf;
-  __syncthreads();
 }
 
 // Where each segment's particles end up after the last pass: live ones
@@ -1777,84 +1751,94 @@ __device__ void seg_layout(const uint32_t* __restrict__ segCount, const uint32_t
   if (threadIdx.x == 0) *nLive = cA;
 }
 
-// Per digit (one workgroup each): counts[d][t] becomes the exclusive prefix
-// over the earlier tiles of t's segment; the segment's total goes to
-// segTot[seg][d] at its last tile.  With `layout`, workgroup RADIX lays out
-// the segments' live and removed particles instead.
-template <int BITS, int ITEMS>
-__global__ __launch_bounds__(256) void k_seg_rowscan(uint32_t* __restrict__ counts, uint32_t stride,
-                                                     const uint32_t* __restrict__ tseg,
-                                                     const uint32_t* __restrict__ nTiles,
-                                                     uint32_t* __restrict__ segTot, int layout,
-                                                     const uint32_t* __restrict__ segCount,
-                                                     const uint32_t* __restrict__ repDead, int R,
-                                                     uint32_t* __restrict__ liveStart,
-                                                     uint32_t* __restrict__ deadStart, uint32_t* __restrict__ nLive) {
+// Exclusive scans of the digit counts over each segment's tiles: counts[t][d]
+// becomes the sum over the segment's tiles before t, segTot[seg][d] the
+// segment's total.  Workgroup r < R scans segment r when it has at most
+// kSegScanLong tiles: a thread per digit walks the tiles in order, every row
+// read whole by the workgroup.  Workgroup R + d scans digit d of the longer
+// segments (none in most ensembles: then it returns at once) by block scans.
+// With `layout`, workgroup R + RADIX lays out the segments' live and removed
+// particles.
+template <int BITS>
+__global__ __launch_bounds__(256) void k_seg_scan(uint32_t* __restrict__ counts,
+                                                  const uint32_t* __restrict__ segFirst,
+                                                  const uint32_t* __restrict__ segCount, int R, uint32_t tileSize,
+                                                  const uint32_t* __restrict__ nTiles,
+                                                  const uint32_t* __restrict__ nLong, uint32_t* __restrict__ segTot,
+                                                  int layout, const uint32_t* __restrict__ repDead,
+                                                  uint32_t* __restrict__ liveStart, uint32_t* __restrict__ deadStart,
+                                                  uint32_t* __restrict__ nLive) {
   constexpr int RADIX = 1 << BITS;
-  constexpr int PER = 8;
-  __shared__ uint32_t wv[4], wf[4];
-  __shared__ uint32_t s_lastV, s_lastF;
-  const uint32_t d = blockIdx.x;
-  if (d >= (uint32_t)RADIX) {
-    if (layout) seg_layout(segCount, repDead, R, liveStart, deadStart, nLive, wv, &s_lastV);
+  constexpr int DPT = (RADIX + 255) / 256;
+  constexpr int BATCH = 16;  // tile rows requested together
+  constexpr int PER = 8;     // tiles per thread in a long segment's block scan
+  __shared__ uint32_t wsum[4];
+  __shared__ uint32_t s_tot;
+  const uint32_t g = blockIdx.x;
+  const uint32_t T = *nTiles;
+  if (g < (uint32_t)R) {
+    const uint32_t nt = (segCount[g] + tileSize - 1u) / tileSize;
+    const uint32_t t0 = segFirst[g];
+    // a table that does not fit was reported by k_seg_tiles
+    if (nt == 0u || nt > kSegScanLong || t0 >= T || nt > T - t0) return;
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+      const uint32_t d = threadIdx.x + 256u * q;
+      if (d >= (uint32_t)RADIX) break;
+      uint32_t* col = counts + (size_t)t0 * RADIX + d;
+      uint32_t run = 0;
+      for (uint32_t k0 = 0; k0 < nt; k0 += BATCH) {
+        uint32_t v[BATCH];
+#pragma unroll
+        for (int i = 0; i < BATCH; ++i) v[i] = k0 + i < nt ? col[(size_t)(k0 + i) * RADIX] : 0u;
+#pragma unroll
+        for (int i = 0; i < BATCH; ++i) {
+          if (k0 + i < nt) col[(size_t)(k0 + i) * RADIX] = run;
+          run += v[i];
+        }
+      }
+      segTot[(size_t)g * RADIX + d] = run;
+    }
     return;
   }
-  const uint32_t T = *nTiles;
-  uint32_t* row = counts + (size_t)d * stride;
-  uint32_t carry = 0;  // running sum of the segment open at the chunk start
-  for (uint32_t b = 0; b < T; b += 256 * PER) {
-    const uint32_t e0 = b + threadIdx.x * PER;
-    uint32_t v[PER], sg[PER];
-    bool h[PER + 1];
+  if (g < (uint32_t)R + RADIX) {
+    if (*nLong == 0u) return;
+    const uint32_t d = g - (uint32_t)R;
+    for (int r = 0; r < R; ++r) {
+      const uint32_t nt = (segCount[r] + tileSize - 1u) / tileSize;
+      const uint32_t t0 = segFirst[r];
+      if (nt <= kSegScanLong || t0 >= T || nt > T - t0) continue;
+      uint32_t* col = counts + (size_t)t0 * RADIX + d;
+      uint32_t carry = 0;
+      for (uint32_t b = 0; b < nt; b += 256u * PER) {
+        const uint32_t e0 = b + threadIdx.x * PER;
+        uint32_t v[PER], sum = 0;
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const bool ok = e0 + i < T;
-      v[i] = ok ? row[e0 + i] : 0u;
-      const uint32_t ts = ok ? tseg[e0 + i] : kSegHead;
-      h[i] = (ts & kSegHead) != 0u;
-      sg[i] = ts & ~kSegHead;
-    }
-    h[PER] = e0 + PER < T ? (tseg[e0 + PER] & kSegHead) != 0u : true;
-    uint32_t aggV = 0;
-    bool aggF = false;
+        for (int i = 0; i < PER; ++i) {
+          v[i] = e0 + i < nt ? col[(size_t)(e0 + i) * RADIX] : 0u;
+          sum += v[i];
+        }
+        uint32_t run = carry + block_exscan256(sum, wsum, &s_tot);
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      if (h[i]) {
-        aggV = 0;
-        aggF = true;
+        for (int i = 0; i < PER; ++i) {
+          if (e0 + i < nt) col[(size_t)(e0 + i) * RADIX] = run;
+          run += v[i];
+        }
+        carry += s_tot;
+        __syncthreads();  // s_tot is rewritten by the next chunk
       }
-      aggV += v[i];
+      if (threadIdx.x == 0) segTot[(size_t)r * RADIX + d] = carry;
     }
-    uint32_t incV, exV;
-    bool incF, exF;
-    block_segscan256(aggV, aggF, incV, incF, exV, exF, wv, wf);
-    uint32_t run = exF ? exV : carry + exV;
-#pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const uint32_t e = e0 + i;
-      if (h[i]) run = 0;
-      if (e < T) {
-        row[e] = run;
-        if (h[i + 1]) segTot[(size_t)sg[i] * RADIX + d] = run + v[i];  // the segment's last tile
-      }
-      run += v[i];
-    }
-    // the last thread's inclusive pair is the chunk's
-    if (threadIdx.x == 255) {
-      s_lastV = incV;
-      s_lastF = incF ? 1u : 0u;
-    }
-    __syncthreads();
-    carry = s_lastF ? s_lastV : carry + s_lastV;
-    __syncthreads();
+    return;
   }
+  if (layout) seg_layout(segCount, repDead, R, liveStart, deadStart, nLive, wsum, &s_tot);
 }
 
 template <int BITS, int ITEMS, bool GATHER, bool LAST>
 __global__ __launch_bounds__(256) void k_seg_down(const uint32_t* __restrict__ keysIn,
                                                   const uint32_t* __restrict__ valsIn,
                                                   uint32_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut,
-                                                  const uint32_t* __restrict__ counts, uint32_t stride,
+                                                  const uint32_t* __restrict__ counts,
                                                   const uint2* __restrict__ tpos,
                                                   const uint32_t* __restrict__ tseg,
                                                   const uint32_t* __restrict__ nTiles,
@@ -1924,7 +1908,7 @@ __global__ __launch_bounds__(256) void k_seg_down(const uint32_t* __restrict__ k
       if (d < (uint32_t)RADIX) {
         // removed particles, and only they, have the top digit in the last pass
         const uint32_t b0 = (LAST && d == (uint32_t)RADIX - 1u) ? deadBase : liveBase + run;
-        s_base[d] = b0 + counts[(size_t)d * stride + tile];
+        s_base[d] = b0 + counts[(size_t)tile * RADIX + d];
       }
       run += tv[q];
     }
@@ -2128,7 +2112,7 @@ static uint32_t seg_max_tiles(uint32_t maxN, int R) { return cdiv(maxN, 256u * k
 size_t seg_sort_words(uint32_t maxN, int R, int maxBits) {
   const size_t radix = (size_t)1 << maxBits;
   const size_t mt = seg_max_tiles(maxN, R);
-  return 3 * mt + 64 + radix * (mt + 1) + (size_t)R * radix + 3 * (size_t)R + 64;
+  return 3 * mt + 64 + radix * (mt + 1) + (size_t)R * radix + 4 * (size_t)R + 64;
 }
 
 void seg_sort_digits(uint32_t maxGridBits, int& lowBits, int& highBits) {
@@ -2150,8 +2134,8 @@ void seg_sort_digits(uint32_t maxGridBits, int& lowBits, int& highBits) {
 namespace {
 struct SegScratch {
   uint2* tpos;
-  uint32_t *tseg, *nTiles, *nLive, *counts, *segTot, *repDead, *liveStart, *deadStart;
-  uint32_t maxTiles, stride;
+  uint32_t *tseg, *nTiles, *nLive, *nLong, *counts, *segTot, *repDead, *liveStart, *deadStart, *segFirst;
+  uint32_t maxTiles;
 };
 }  // namespace
 
@@ -2165,23 +2149,23 @@ static void seg_pass(SortBuffers& b, const SegSortArgs& g, const SegScratch& s, 
   constexpr int RADIX = 1 << BITS;
   const dim3 grid(s.maxTiles), blk(256);
   hipLaunchKernelGGL((k_seg_up<BITS, kSegItems, FIRST>), grid, blk, 0, stream, b.keys, s.tpos, s.tseg, s.nTiles,
-                     g.bucketOff, g.deadKey, s.counts, s.stride, shift, s.repDead);
-  hipLaunchKernelGGL((k_seg_rowscan<BITS, kSegItems>), dim3(RADIX + (FIRST ? 1 : 0)), blk, 0, stream, s.counts,
-                     s.stride, s.tseg, s.nTiles, s.segTot, FIRST ? 1 : 0, g.segCount, s.repDead, g.R, s.liveStart,
-                     s.deadStart, s.nLive);
+                     g.bucketOff, g.deadKey, s.counts, shift, s.repDead);
+  hipLaunchKernelGGL((k_seg_scan<BITS>), dim3((uint32_t)g.R + RADIX + (FIRST ? 1u : 0u)), blk, 0, stream, s.counts,
+                     s.segFirst, g.segCount, g.R, 256u * kSegItems, s.nTiles, s.nLong, s.segTot, FIRST ? 1 : 0,
+                     s.repDead, s.liveStart, s.deadStart, s.nLive);
   const int idv = identityVals ? 1 : 0;
   if (!LAST)
     hipLaunchKernelGGL((k_seg_down<BITS, kSegItems, false, false>), grid, blk, 0, stream, b.keys, b.vals, b.keysAlt,
-                       b.valsAlt, s.counts, s.stride, s.tpos, s.tseg, s.nTiles, g.bucketOff, g.deadKey, g.segOff,
+                       b.valsAlt, s.counts, s.tpos, s.tseg, s.nTiles, g.bucketOff, g.deadKey, g.segOff,
                        s.segTot, s.liveStart, s.deadStart, s.nLive, shift, idv, nullptr, nullptr, dN, b.status);
   else if (b.gatherDst)
     hipLaunchKernelGGL((k_seg_down<BITS, kSegItems, true, true>), grid, blk, 0, stream, b.keys, b.vals, b.keysAlt,
-                       b.valsAlt, s.counts, s.stride, s.tpos, s.tseg, s.nTiles, g.bucketOff, g.deadKey, g.segOff,
+                       b.valsAlt, s.counts, s.tpos, s.tseg, s.nTiles, g.bucketOff, g.deadKey, g.segOff,
                        s.segTot, s.liveStart, s.deadStart, s.nLive, shift, idv, b.gatherSrc, b.gatherDst, dN,
                        b.status);
   else
     hipLaunchKernelGGL((k_seg_down<BITS, kSegItems, false, true>), grid, blk, 0, stream, b.keys, b.vals, b.keysAlt,
-                       b.valsAlt, s.counts, s.stride, s.tpos, s.tseg, s.nTiles, g.bucketOff, g.deadKey, g.segOff,
+                       b.valsAlt, s.counts, s.tpos, s.tseg, s.nTiles, g.bucketOff, g.deadKey, g.segOff,
                        s.segTot, s.liveStart, s.deadStart, s.nLive, shift, idv, nullptr, nullptr, dN, b.status);
 }
 
@@ -2190,7 +2174,6 @@ static hipError_t seg_sort_run(SortBuffers& b, const SegSortArgs& g, const uint3
                                bool identityVals, hipStream_t stream) {
   SegScratch s;
   s.maxTiles = seg_max_tiles(maxN, g.R);
-  s.stride = s.maxTiles + 1;
   uint32_t* w = g.scratch;
   s.tpos = (uint2*)w;
   w += 2 * (size_t)s.maxTiles;
@@ -2198,17 +2181,19 @@ static hipError_t seg_sort_run(SortBuffers& b, const SegSortArgs& g, const uint3
   w += s.maxTiles;
   s.nTiles = w;
   s.nLive = w + 1;
+  s.nLong = w + 2;
   w += 64;
   constexpr int MB = LB > HB ? LB : HB;
   s.counts = w;
-  w += ((size_t)1 << MB) * s.stride;
+  w += ((size_t)1 << MB) * (s.maxTiles + 1);
   s.segTot = w;
   w += (size_t)g.R << MB;
   s.repDead = w;
   s.liveStart = w + g.R;
   s.deadStart = w + 2 * (size_t)g.R;
+  s.segFirst = w + 3 * (size_t)g.R;
   hipLaunchKernelGGL(k_seg_tiles, dim3(1), dim3(1024), 0, stream, g.segOff, g.segCount, g.R, 256u * kSegItems,
-                     s.maxTiles, s.tpos, s.tseg, s.nTiles, s.repDead, dN, b.status);
+                     s.maxTiles, s.tpos, s.tseg, s.nTiles, s.segFirst, s.nLong, s.repDead, dN, b.status);
   seg_pass<LB, true, false>(b, g, s, dN, 0, identityVals, stream);
   std::swap(b.keys, b.keysAlt);
   std::swap(b.vals, b.valsAlt);

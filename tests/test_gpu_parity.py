@@ -524,6 +524,33 @@ def test_segmented_sort_agrees(smd, gpu, big):
     assert sum(len(t) for _, t in res[0]) > 0
 
 
+def test_segmented_sort_long_segments(smd, gpu):
+    """Replicas of more than 64 sort tiles (4096 keys each) have their tile
+    counts scanned per digit instead of by one workgroup per replica: two such
+    replicas (300k and 530k particles) between short ones give the global
+    sort's particles bit for bit."""
+    reps = []
+    for k, n in enumerate((20000, 300000, 5000, 530000, 70000)):
+        spec, pos, types = configs.c3(n=n, n_blobs=8, sigma=2.0, n_streams=4096, grid_bits=(6, 6, 5), cell=0.2,
+                                      L=40.0, seed=700 + k)
+        reps.append((spec, pos, types))
+    banner("segmented sort, long segments")
+    res = []
+    for seg in (1, 0):
+        e = smd.Engine(0)
+        for spec, pos, types in reps:
+            e.add_replica(spec, pos, types)
+        e.finalize()
+        e.set_knob(smd.KNOB_SEGMENTED_SORT, seg)
+        assert e.sort_plan()[1] == (seg == 1)
+        e.step(4)
+        res.append([e.get(r) for r in range(len(reps))])
+        e.close()
+    for r, ((p1, t1), (p0, t0)) in enumerate(zip(*res)):
+        assert np.array_equal(t1, t0) and np.array_equal(p1.view(np.uint32), p0.view(np.uint32)), r
+    assert len(res[0][3][1]) > 64 * 4096
+
+
 def test_segmented_sort_many_replicas(smd, gpu):
     """More replicas than one workgroup's lanes (1100 small replicas, some
     empty): the segmented sort's tile table and layout scans loop over chunks
