@@ -49,7 +49,12 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def stage_bytes(name, n, S, nb, passes):
+# Sorted records carry their particle's index (no sorted values written) while
+# every index fits in 27 bits.
+PACKED_INDEX_MAX = 1 << 27
+
+
+def stage_bytes(name, n, S, nb, passes, packed=True):
     """Algorithmic bytes of one launch of each stage for n particles, S RNG
     streams, nb buckets (compulsory reads + writes, each datum once)."""
     if name == "producer":
@@ -58,8 +63,9 @@ def stage_bytes(name, n, S, nb, passes):
         return 36 * n + 172 * S
     if name == "sort":
         # per digit pass: count (key 4) + scatter (key, value 8 in, 8 out); first pass values implicit;
-        # the last pass also copies the packed record of each particle into sorted order (16 + 16)
-        return 20 * n * passes - 4 * n + 32 * n
+        # the last pass also copies the packed record of each particle into sorted order (16 + 16),
+        # with the particle's index in the record instead of a sorted value written (4 less)
+        return 20 * n * passes - 4 * n + 32 * n - (4 * n if packed else 0)
     if name == "first":
         return 4 * (nb + 1) + 4 * n
     if name == "candidates":
@@ -164,9 +170,10 @@ def roofline(e, cfg, nprof=5, streams=None, buckets=None):
     S = streams if streams is not None else lay["total_streams"]
     nb = buckets if buckets is not None else lay["total_buckets"]
     passes = e.sort_plan()[0]  # two for an ensemble sorted replica by replica
+    packed = lay["capacity"] <= PACKED_INDEX_MAX
     rows = {}
     for k, us in prof.items():
-        b = stage_bytes(k, n, S, nb, passes)
+        b = stage_bytes(k, n, S, nb, passes, packed)
         rows[k] = dict(us=us, bytes=b, gbs=(b / (us * 1e-6) / 1e9) if us > 0 else 0.0)
     dom = max((k for k in rows if k in SINGLE_KERNEL), key=lambda k: rows[k]["us"])
     step_us = sum(r["us"] for r in rows.values())

@@ -406,6 +406,7 @@ smd_status smd_sort_pairs(int device, uint32_t* keys, uint32_t* values, unsigned
   b.gatherDst = nullptr;
   b.gatherDeadKey = 0;
   b.sizeHint = 0;
+  b.packIndex = 0;
   uint32_t* dN = nullptr;
   CAPI_HIPCHK(hipMalloc((void**)&b.keys, (size_t)n * 4));
   CAPI_HIPCHK(hipMalloc((void**)&b.vals, (size_t)n * 4));

@@ -651,6 +651,9 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
     sb.gatherDst = dSortedPT;
     sb.gatherDeadKey = deadKey;
     sb.sizeHint = (uint32_t)std::min<uint64_t>(knownN, 0xFFFFFFFFu);
+    // the sort's last pass puts each particle's index into its sorted record
+    // when the gather is fused into it and every index fits
+    sb.packIndex = (sb.algo == 1 && cap <= kMaxPackedIndex) ? 1 : 0;
     HIPCHK(sort_keys(maxN));
     mark(2);
     HIPCHK(launch_first(sb.keys, dN, maxN, nb, dFirst, dLive, dTables, slotsMode ? slotBits : 0u, R, stream));
@@ -678,6 +681,7 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
     ra.cand = dCand;
     ra.sortedPT = dSortedPT;
     ra.vals = sb.vals;
+    ra.packed = sb.packIndex;
     ra.pt = dPT[cur];
     ra.w = dW[cur];
     ra.repDeaths = dRepDeaths;
@@ -1149,6 +1153,7 @@ smd_status Engine::debug_stage(int stage, float* pos4, int* types, uint32_t* key
     sb.gatherDst = dSortedPT;
     sb.gatherDeadKey = deadKey;
     sb.sizeHint = (uint32_t)std::min<uint64_t>(knownN, 0xFFFFFFFFu);
+    sb.packIndex = 0;  // the sorted values are returned
     HIPCHK(sort_keys(maxN));
     HIPCHK(launch_first(sb.keys, dN, maxN, nb, dFirst, dLive, dTables, slotsMode ? slotBits : 0u, R, stream));
   }
@@ -1177,6 +1182,7 @@ smd_status Engine::debug_stage(int stage, float* pos4, int* types, uint32_t* key
     ra.cand = dCand;
     ra.sortedPT = dSortedPT;
     ra.vals = sb.vals;
+    ra.packed = 0;
     ra.pt = dPT[cur];
     ra.w = dW[cur];
     ra.repDeaths = dRepDeaths;

@@ -425,7 +425,8 @@ __global__ __launch_bounds__(256) void k_sort_down(const uint32_t* __restrict__ 
                                                    const uint32_t* __restrict__ totals,
                                                    const uint32_t* __restrict__ dN, uint32_t stride, int shift,
                                                    int identityVals, const float4* __restrict__ gatherSrc,
-                                                   float4* __restrict__ gatherDst, uint32_t gatherDeadKey) {
+                                                   float4* __restrict__ gatherDst, uint32_t gatherDeadKey,
+                                                   int packIndex) {
   constexpr int RADIX = 1 << BITS;
   constexpr int DPT = (RADIX + 255) / 256;  // digits per thread in the per-digit loops
   constexpr uint32_t TILE = 256u * ITEMS;
@@ -552,8 +553,15 @@ __global__ __launch_bounds__(256) void k_sort_down(const uint32_t* __restrict__ 
     for (int q = 0; q < ITEMS; ++q) {
       if (t + q * 256u < tileN) {
         keysOut[dd[q]] = dk[q];
-        valsOut[dd[q]] = dv[q];
-        if (dk[q] < gatherDeadKey) gatherDst[dd[q]] = rec[q];
+        if (packIndex) {
+          // the record carries its particle's index above the type bits
+          if (dk[q] < gatherDeadKey)
+            gatherDst[dd[q]] = make_float4(rec[q].x, rec[q].y, rec[q].z,
+                                           __int_as_float(pack_index_type(dv[q], __float_as_int(rec[q].w))));
+        } else {
+          valsOut[dd[q]] = dv[q];
+          if (dk[q] < gatherDeadKey) gatherDst[dd[q]] = rec[q];
+        }
       }
     }
     return;
@@ -898,17 +906,22 @@ __device__ __forceinline__ void consider(const ReplicaTables& T, const float4 q,
   }
 }
 
+// A sorted record with any index bits above its type cleared.
+__device__ __forceinline__ float4 type_only(const float4 q) {
+  return make_float4(q.x, q.y, q.z, __int_as_float(__float_as_int(q.w) & kTypeMask));
+}
+
 __device__ __forceinline__ void scan_range(const ReplicaTables& T, const float4* __restrict__ spt, uint32_t a,
                                            uint32_t b, uint32_t j, float px, float py, float pz, int tj,
                                            uint32_t mask, float maxB2, int& best, float& bestD2) {
   uint32_t k = a;
   for (; k + 2 <= b; k += 2) {  // two loads in flight
-    const float4 q0 = spt[k];
-    const float4 q1 = spt[k + 1];
+    const float4 q0 = type_only(spt[k]);
+    const float4 q1 = type_only(spt[k + 1]);
     consider(T, q0, k, j, px, py, pz, tj, mask, maxB2, best, bestD2);
     consider(T, q1, k + 1, j, px, py, pz, tj, mask, maxB2, best, bestD2);
   }
-  if (k < b) consider(T, spt[k], k, j, px, py, pz, tj, mask, maxB2, best, bestD2);
+  if (k < b) consider(T, type_only(spt[k]), k, j, px, py, pz, tj, mask, maxB2, best, bestD2);
 }
 
 // Face-aware neighbour search.  Every binding radius is at most the cell
@@ -933,7 +946,7 @@ constexpr float kFaceMargin = 1.0f / 64.0f;
 __device__ __forceinline__ void wave_search(const ReplicaTables& T, const CandArgs& a, uint32_t j, uint32_t wbase,
                                             uint32_t lane, bool active, uint32_t key, const float4 p, int& best,
                                             bool uniform) {
-  const int tj = __float_as_int(p.w);
+  const int tj = __float_as_int(p.w) & kTypeMask;
   // one replica in the wave: the per-type entries sit one per lane (lane t
   // holds type t's) and reach each lane by a shuffle, so no load waits for
   // the particle's type; otherwise each lane loads its own replica's entry
@@ -1002,8 +1015,9 @@ __device__ __forceinline__ void wave_search(const ReplicaTables& T, const CandAr
     int t = (int)lane - s + o;
     if (t >= len) t -= len;
     const int src = take ? s + t : (int)lane;
+    // the neighbour's type (without the index bits) stands in for its w
     const float4 q = make_float4(__shfl(p.x, src, 64), __shfl(p.y, src, 64), __shfl(p.z, src, 64),
-                                 __shfl(p.w, src, 64));
+                                 __int_as_float(__shfl(tj, src, 64)));
     if (take) consider(T, q, wbase + (uint32_t)src, j, p.x, p.y, p.z, tj, mask, maxB2, best, bestD2);
   }
   // the wave's first and last keys, read while every lane is still active
@@ -1039,7 +1053,7 @@ __device__ __forceinline__ void wave_search(const ReplicaTables& T, const CandAr
     const uint32_t rb[5] = {spillLo ? wbase : 0u, spillHi ? oHi : 0u, fb[0], fb[1], fb[2]};
     float4 q0[5];
 #pragma unroll
-    for (int c = 0; c < 5; ++c) q0[c] = ra[c] < rb[c] ? a.sortedPT[ra[c]] : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int c = 0; c < 5; ++c) q0[c] = ra[c] < rb[c] ? type_only(a.sortedPT[ra[c]]) : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
     for (int c = 0; c < 5; ++c) {
       if (ra[c] < rb[c]) {
@@ -1144,7 +1158,7 @@ __global__ __launch_bounds__(kBS) void k_resolve(ResolveArgs a, const ReplicaTab
       if (a.cand[c] != (int)j) continue;
       const float4 p1 = a.sortedPT[j];
       const float4 p2 = a.sortedPT[c];
-      const int t1 = __float_as_int(p1.w), t2 = __float_as_int(p2.w);
+      const int t1 = __float_as_int(p1.w) & kTypeMask, t2 = __float_as_int(p2.w) & kTypeMask;
       const int rx = T.pairRxn[t1 * kMaxTypes + t2];
       if (!loaded) {
         mt_load<kBS>(m, &lds_mt[threadIdx.x], a.mtState, a.mtPar, a.Stot, g);
@@ -1165,7 +1179,8 @@ __global__ __launch_bounds__(kBS) void k_resolve(ResolveArgs a, const ReplicaTab
       const float ch = rand_real(mt_draw<kBS>(m)) * 6.28318548f;
       float v1[3], v2[3];
       rotate_offsets(th, ph, ch, T.prodPos[rx], v1, v2);
-      const uint32_t mol1 = a.vals[j], mol2 = a.vals[c];
+      const uint32_t mol1 = a.packed ? unpack_index(p1.w) : a.vals[j];
+      const uint32_t mol2 = a.packed ? unpack_index(p2.w) : a.vals[c];
       float n1[3] = {rp[0] + v1[0], rp[1] + v1[1], rp[2] + v1[2]};
       float n2[3] = {rp[0] + v2[0], rp[1] + v2[1], rp[2] + v2[2]};
   #pragma unroll
@@ -1850,7 +1865,7 @@ __global__ __launch_bounds__(256) void k_seg_down(const uint32_t* __restrict__ k
                                                   const uint32_t* __restrict__ nLive, int shift, int identityVals,
                                                   const float4* __restrict__ gatherSrc,
                                                   float4* __restrict__ gatherDst, const uint32_t* __restrict__ dN,
-                                                  uint32_t* __restrict__ status) {
+                                                  uint32_t* __restrict__ status, int packIndex) {
   constexpr int RADIX = 1 << BITS;
   constexpr int DPT = (RADIX + 255) / 256;
   constexpr uint32_t TILE = 256u * ITEMS;
@@ -1993,8 +2008,15 @@ __global__ __launch_bounds__(256) void k_seg_down(const uint32_t* __restrict__ k
           continue;
         }
         keysOut[dd[q]] = dk[q];
-        valsOut[dd[q]] = dv[q];
-        if (dk[q] < deadKey) gatherDst[dd[q]] = rec[q];
+        if (packIndex) {
+          // the record carries its particle's index above the type bits
+          if (dk[q] < deadKey)
+            gatherDst[dd[q]] = make_float4(rec[q].x, rec[q].y, rec[q].z,
+                                           __int_as_float(pack_index_type(dv[q], __float_as_int(rec[q].w))));
+        } else {
+          valsOut[dd[q]] = dv[q];
+          if (dk[q] < deadKey) gatherDst[dd[q]] = rec[q];
+        }
       }
     }
   } else {
@@ -2157,16 +2179,16 @@ static void seg_pass(SortBuffers& b, const SegSortArgs& g, const SegScratch& s, 
   if (!LAST)
     hipLaunchKernelGGL((k_seg_down<BITS, kSegItems, false, false>), grid, blk, 0, stream, b.keys, b.vals, b.keysAlt,
                        b.valsAlt, s.counts, s.tpos, s.tseg, s.nTiles, g.bucketOff, g.deadKey, g.segOff,
-                       s.segTot, s.liveStart, s.deadStart, s.nLive, shift, idv, nullptr, nullptr, dN, b.status);
+                       s.segTot, s.liveStart, s.deadStart, s.nLive, shift, idv, nullptr, nullptr, dN, b.status, 0);
   else if (b.gatherDst)
     hipLaunchKernelGGL((k_seg_down<BITS, kSegItems, true, true>), grid, blk, 0, stream, b.keys, b.vals, b.keysAlt,
                        b.valsAlt, s.counts, s.tpos, s.tseg, s.nTiles, g.bucketOff, g.deadKey, g.segOff,
                        s.segTot, s.liveStart, s.deadStart, s.nLive, shift, idv, b.gatherSrc, b.gatherDst, dN,
-                       b.status);
+                       b.status, b.packIndex);
   else
     hipLaunchKernelGGL((k_seg_down<BITS, kSegItems, false, true>), grid, blk, 0, stream, b.keys, b.vals, b.keysAlt,
                        b.valsAlt, s.counts, s.tpos, s.tseg, s.nTiles, g.bucketOff, g.deadKey, g.segOff,
-                       s.segTot, s.liveStart, s.deadStart, s.nLive, shift, idv, nullptr, nullptr, dN, b.status);
+                       s.segTot, s.liveStart, s.deadStart, s.nLive, shift, idv, nullptr, nullptr, dN, b.status, 0);
 }
 
 template <int LB, int HB>
@@ -2250,10 +2272,11 @@ static hipError_t sort3k_pass(SortBuffers& b, const uint32_t* dN, uint32_t maxTi
   if (gather)
     hipLaunchKernelGGL((k_sort_down<BITS, ITEMS, true>), dim3(maxTiles), dim3(256), 0, stream, b.keys, b.vals,
                        b.keysAlt, b.valsAlt, b.counts, totals, dN, stride, shift, idv ? 1 : 0, b.gatherSrc,
-                       b.gatherDst, b.gatherDeadKey);
+                       b.gatherDst, b.gatherDeadKey, b.packIndex);
   else
     hipLaunchKernelGGL((k_sort_down<BITS, ITEMS, false>), dim3(maxTiles), dim3(256), 0, stream, b.keys, b.vals,
-                       b.keysAlt, b.valsAlt, b.counts, totals, dN, stride, shift, idv ? 1 : 0, nullptr, nullptr, 0u);
+                       b.keysAlt, b.valsAlt, b.counts, totals, dN, stride, shift, idv ? 1 : 0, nullptr, nullptr, 0u,
+                       0);
   return hipGetLastError();
 }
 

@@ -756,6 +756,7 @@ int radixSort(unsigned int* keys, unsigned int* values, unsigned int* tempKeys, 
   b.gatherDst = nullptr;
   b.gatherDeadKey = 0;
   b.sizeHint = 0;
+  b.packIndex = 0;
   LCHK(hipMemcpy(dN, &numElements, sizeof(uint32_t), hipMemcpyHostToDevice));
   LCHK(launch_sort(b, dN, numElements, keyBits, rbits, false, nullptr));
   if (b.keys != keys) {

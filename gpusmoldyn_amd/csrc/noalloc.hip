@@ -111,6 +111,7 @@ extern "C" int smdRadixSortNoAlloc(unsigned int* keys, unsigned int* values, uns
   b.gatherDst = nullptr;
   b.gatherDeadKey = 0;
   b.sizeHint = 0;
+  b.packIndex = 0;
   // same stream as radixSort (the null stream), asynchronous like it
   hipLaunchKernelGGL(k_store_count, dim3(1), dim3(64), 0, nullptr, dN, (uint32_t)numElements);
   hipError_t e = hipGetLastError();

@@ -296,6 +296,12 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
   return (lane == 0) ? 0ull : ((~0ull) >> (64u - lane));
 }
 
+__device__ __forceinline__ int pack_index_type(uint32_t index, int type) {
+  return (int)((index << kIndexShift) | (uint32_t)type);
+}
+
+__device__ __forceinline__ uint32_t unpack_index(float w) { return __float_as_uint(w) >> kIndexShift; }
+
 // Number of set bits of m at lanes below this one (v_mbcnt).
 __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));

@@ -43,6 +43,10 @@ struct SortBuffers {
   float4* gatherDst;
   uint32_t gatherDeadKey;
   uint32_t sizeHint;  // expected key count (tile size choice only), 0: maxN
+  // with the gather fused into the last pass (algo 1): 1 puts each particle's
+  // index into its sorted record (w = index << kIndexShift | type, indices
+  // below kMaxPackedIndex) instead of writing the sorted values
+  int packIndex;
 };
 
 // Segments of an ensemble's replica-major keys for the segmented sort:
@@ -88,6 +92,7 @@ struct ResolveArgs {
   const int* cand;
   const float4* sortedPT;
   const uint32_t* vals;
+  int packed;  // sortedPT carries the particles' indices (vals unused)
   float4* pt;
   float* w;
   uint32_t* repDeaths;

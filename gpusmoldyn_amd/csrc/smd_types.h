@@ -16,6 +16,13 @@ constexpr int kMtWords = 20;     // 19 state words + iState, stored word-major (
 constexpr int kMtParamSets = 32768;  // records in MersenneTwister.dat (MersenneTwister.h:31)
 constexpr int kGaussDim = 4096;  // particles_kernel.cuh:43
 constexpr int kCompactTile = 1024;   // elements per compaction tile (4 waves x 4 chunks of 64)
+// Sorted particle records may carry the particle's index above its type:
+// w bits = index << kIndexShift | type (types stay below 2^kIndexShift), for
+// indices below kMaxPackedIndex.
+constexpr int kIndexShift = 5;
+constexpr int kTypeMask = (1 << kIndexShift) - 1;
+constexpr uint32_t kMaxPackedIndex = 1u << (32 - kIndexShift);
+static_assert(kMaxTypes <= (1 << kIndexShift), "types fit below the packed index");
 
 // Boundary behaviour of one face of the simulation box.
 enum Boundary : int32_t {

@@ -465,6 +465,28 @@ def test_sort_kinds_agree(smd, gpu):
     assert np.array_equal(at, bt) and np.array_equal(ap.view(np.uint32), bp.view(np.uint32))
 
 
+def test_sorted_index_unpacked_above_2_27(smd, gpu):
+    """Sorted records carry their particle's index above the type bits while
+    every index fits in 27 bits; an engine with a larger capacity keeps the
+    sorted values instead.  Both give the same particles, bit for bit."""
+    spec, pos, types = w_periodic_reversible()
+    banner("packed sorted index vs sorted values (capacity 2^27 + 4096)")
+    a = _engine(smd, spec, pos, types)
+    b = smd.Engine(0)
+    b.add_replica(spec, pos, types)
+    b.finalize((1 << 27) + 4096)
+    try:
+        a.step(15)
+        b.step(15)
+        ap, at = a.get(0)
+        bp, bt = b.get(0)
+        assert np.array_equal(at, bt) and np.array_equal(ap.view(np.uint32), bp.view(np.uint32))
+        assert len(at) != len(types) or not np.array_equal(at, types)  # reactions happened
+    finally:
+        a.close()
+        b.close()
+
+
 def test_launch_shapes_agree(smd, gpu):
     """Every producer and pair-search workgroup size gives the same particles,
     bit for bit (the autotuner picks among them by time alone)."""
