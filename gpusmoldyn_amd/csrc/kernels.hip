@@ -892,8 +892,12 @@ __device__ __forceinline__ int replica_of_bucket(const uint32_t* __restrict__ bu
 __device__ __forceinline__ void consider(const ReplicaTables& T, const float4 q, uint32_t k, uint32_t j, float px,
                                          float py, float pz, int tj, uint32_t mask, float maxB2, int& best,
                                          float& bestD2) {
-  const int tk = __float_as_int(q.w);
-  if (k == j || tk <= 0 || tk >= kMaxTypes || !((mask >> tk) & 1u)) return;
+  // Sorted records are live particles (types 1 to kMaxTypes - 1; no partner
+  // mask has a bit for type 0) and may carry their index above the type bits:
+  // the bit extract (v_bfe_u32) reads only the low five bits of its offset.
+  const uint32_t bits = __float_as_uint(q.w);
+  if (k == j || !__builtin_amdgcn_ubfe(mask, bits, 1u)) return;
+  const int tk = (int)(bits & kTypeMask);
   const float dx = min_image(px - q.x, T, 0);
   const float dy = min_image(py - q.y, T, 1);
   const float dz = min_image(pz - q.z, T, 2);
@@ -906,22 +910,17 @@ __device__ __forceinline__ void consider(const ReplicaTables& T, const float4 q,
   }
 }
 
-// A sorted record with any index bits above its type cleared.
-__device__ __forceinline__ float4 type_only(const float4 q) {
-  return make_float4(q.x, q.y, q.z, __int_as_float(__float_as_int(q.w) & kTypeMask));
-}
-
 __device__ __forceinline__ void scan_range(const ReplicaTables& T, const float4* __restrict__ spt, uint32_t a,
                                            uint32_t b, uint32_t j, float px, float py, float pz, int tj,
                                            uint32_t mask, float maxB2, int& best, float& bestD2) {
   uint32_t k = a;
   for (; k + 2 <= b; k += 2) {  // two loads in flight
-    const float4 q0 = type_only(spt[k]);
-    const float4 q1 = type_only(spt[k + 1]);
+    const float4 q0 = spt[k];
+    const float4 q1 = spt[k + 1];
     consider(T, q0, k, j, px, py, pz, tj, mask, maxB2, best, bestD2);
     consider(T, q1, k + 1, j, px, py, pz, tj, mask, maxB2, best, bestD2);
   }
-  if (k < b) consider(T, type_only(spt[k]), k, j, px, py, pz, tj, mask, maxB2, best, bestD2);
+  if (k < b) consider(T, spt[k], k, j, px, py, pz, tj, mask, maxB2, best, bestD2);
 }
 
 // Face-aware neighbour search.  Every binding radius is at most the cell
@@ -1053,7 +1052,7 @@ __device__ __forceinline__ void wave_search(const ReplicaTables& T, const CandAr
     const uint32_t rb[5] = {spillLo ? wbase : 0u, spillHi ? oHi : 0u, fb[0], fb[1], fb[2]};
     float4 q0[5];
 #pragma unroll
-    for (int c = 0; c < 5; ++c) q0[c] = ra[c] < rb[c] ? type_only(a.sortedPT[ra[c]]) : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int c = 0; c < 5; ++c) q0[c] = ra[c] < rb[c] ? a.sortedPT[ra[c]] : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
     for (int c = 0; c < 5; ++c) {
       if (ra[c] < rb[c]) {

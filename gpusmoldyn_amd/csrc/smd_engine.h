@@ -113,7 +113,7 @@ class Engine {
 
   // launch shapes and the sort's form (identical results for every setting;
   // see set_knob)
-  int producerBlock = 256, pairBlock = 256, segSort = 1;
+  int producerBlock = 256, pairBlock = 64, segSort = 1;
   int segLow = 0, segHigh = 0;       // digit widths of the segmented sort, 0: not applicable
   uint32_t* dSegScratch = nullptr;   // its tile table and counters
   smd_status set_knob(int knob, int value);

@@ -181,8 +181,8 @@ smd_status smd_engine_set_graphs(smd_engine* e, int on);
 /* Launch shapes of the step kernels, for the autotuner: every setting gives
  * the same results bit for bit.  SMD_KNOB_PRODUCER_BLOCK: threads per
  * workgroup of the diffuse/react/hash kernel (128, 256 = default, 512; one
- * thread per random stream); SMD_KNOB_PAIR_BLOCK: of the pair search (64,
- * 128, 256 = default; one thread per sorted particle).  No counterpart in
+ * thread per random stream); SMD_KNOB_PAIR_BLOCK: of the pair search (64 =
+ * default, 128, 256; one thread per sorted particle).  No counterpart in
  * the reference (fixed 256-thread diffuse and 64-thread collide blocks,
  * particleSystemCuda.cu:167, 336). */
 typedef enum smd_knob {
