@@ -910,6 +910,11 @@ __device__ __forceinline__ void consider(const ReplicaTables& T, const float4 q,
   }
 }
 
+// The whole record is in registers here: without this the compiler defers
+// the x component's load into the branch behind the type test, a second
+// memory round trip for every candidate of a partner type.
+__device__ __forceinline__ void whole(const float4& q) { asm volatile("" ::"v"(q.x), "v"(q.y), "v"(q.z), "v"(q.w)); }
+
 __device__ __forceinline__ void scan_range(const ReplicaTables& T, const float4* __restrict__ spt, uint32_t a,
                                            uint32_t b, uint32_t j, float px, float py, float pz, int tj,
                                            uint32_t mask, float maxB2, int& best, float& bestD2) {
@@ -917,10 +922,16 @@ __device__ __forceinline__ void scan_range(const ReplicaTables& T, const float4*
   for (; k + 2 <= b; k += 2) {  // two loads in flight
     const float4 q0 = spt[k];
     const float4 q1 = spt[k + 1];
+    whole(q0);
     consider(T, q0, k, j, px, py, pz, tj, mask, maxB2, best, bestD2);
+    whole(q1);
     consider(T, q1, k + 1, j, px, py, pz, tj, mask, maxB2, best, bestD2);
   }
-  if (k < b) consider(T, spt[k], k, j, px, py, pz, tj, mask, maxB2, best, bestD2);
+  if (k < b) {
+    const float4 q = spt[k];
+    whole(q);
+    consider(T, q, k, j, px, py, pz, tj, mask, maxB2, best, bestD2);
+  }
 }
 
 // Face-aware neighbour search.  Every binding radius is at most the cell
