@@ -130,6 +130,9 @@ __global__ __launch_bounds__(BS) void k_producer(ProducerArgs a, const ReplicaTa
     // removed this step, without drawing.
     const bool kill = a.retire != nullptr && a.retire[rw] != 0u;
     const uint32_t iters = (n > s0) ? (n - s0 + S - 1) / S : 0;
+    // the replica's box, grid and reaction flags in registers for the loop
+    const BoxGeom G = box_geom(T);
+    const int hasUni = T.hasUni, unbindOffsets = T.unbindOffsets, numSpheres = T.numSpheres;
     // Records are requested one pair of particles ahead.
     auto rec = [&](uint32_t it) __attribute__((always_inline)) {
       const uint32_t iq = s + it * S;
@@ -168,7 +171,7 @@ __global__ __launch_bounds__(BS) void k_producer(ProducerArgs a, const ReplicaTa
         float bw = 0.0f;
         if (t <= 0) {
           t = 0;
-        } else if (T.hasUni) {
+        } else if (hasUni) {
           const int cnt = L.uniCount[t];
           const int base = L.uniStart[t];
           for (int j = 0; j < cnt; ++j) {
@@ -181,7 +184,7 @@ __global__ __launch_bounds__(BS) void k_producer(ProducerArgs a, const ReplicaTa
                 const int t2 = L.prodY[rx];
                 if (t2 > 0) {
                   bt = t2;
-                  if (T.unbindOffsets) {
+                  if (unbindOffsets) {
                     const float th = smd_theta(rand_real(mt_draw<BS>(m)));
                     const float ph = rand_real(mt_draw<BS>(m)) * 6.28318548f;
                     const float ch = rand_real(mt_draw<BS>(m)) * 6.28318548f;
@@ -206,11 +209,11 @@ __global__ __launch_bounds__(BS) void k_producer(ProducerArgs a, const ReplicaTa
         uint32_t key = a.deadKey;
         if (t > 0) {
           const float o[3] = {p.x, p.y, p.z};
-          if (!apply_box(T, np)) {
+          if (!apply_box(G, np)) {
             t = 0;
           } else {
-            if (T.numSpheres) apply_spheres(T, o, np);
-            key = bucketOff + grid_hash(T, np[0], np[1], np[2]);
+            if (numSpheres) apply_spheres(T, o, np);
+            key = bucketOff + grid_hash(G, np[0], np[1], np[2]);
           }
         }
         a.pt[i] = make_float4(np[0], np[1], np[2], __int_as_float(t));

@@ -206,9 +206,39 @@ __device__ __forceinline__ float min_image(float d, const ReplicaTables& T, int 
 // ---------------------------------------------------------------------------
 // Box faces (per axis: upper face first, then lower face, as in
 // particles_kernel.cu:73-86) and reflective spheres.  Returns false when an
-// absorbing face removes the particle.
+// absorbing face removes the particle.  Geo is ReplicaTables or BoxGeom.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bool apply_box(const ReplicaTables& T, float* p) {
+
+// The box and grid fields of a replica's tables, copied once into registers
+// by a loop that moves many particles of one replica (read through the
+// scalar cache otherwise, once per particle).
+struct BoxGeom {
+  float lo[3], hi[3], L[3];
+  int32_t face[6];
+  float particleRadius;
+  float origin[3], cellSize[3];
+  uint32_t gbits[3];
+};
+
+__device__ __forceinline__ BoxGeom box_geom(const ReplicaTables& T) {
+  BoxGeom G;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    G.lo[d] = T.lo[d];
+    G.hi[d] = T.hi[d];
+    G.L[d] = T.L[d];
+    G.origin[d] = T.origin[d];
+    G.cellSize[d] = T.cellSize[d];
+    G.gbits[d] = T.gbits[d];
+  }
+#pragma unroll
+  for (int f = 0; f < 6; ++f) G.face[f] = T.face[f];
+  G.particleRadius = T.particleRadius;
+  return G;
+}
+
+template <class Geo>
+__device__ __forceinline__ bool apply_box(const Geo& T, float* p) {
   bool alive = true;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -282,7 +312,8 @@ __device__ __forceinline__ void apply_spheres(const ReplicaTables& T, const floa
 
 // calcGridPos + calcGridHash (particles_kernel.cu:169-186), local to a
 // replica's grid; the caller adds the replica's bucket offset.
-__device__ __forceinline__ uint32_t grid_hash(const ReplicaTables& T, float x, float y, float z) {
+template <class Geo>
+__device__ __forceinline__ uint32_t grid_hash(const Geo& T, float x, float y, float z) {
   const int gx = (int)floorf((x - T.origin[0]) / T.cellSize[0]);
   const int gy = (int)floorf((y - T.origin[1]) / T.cellSize[1]);
   const int gz = (int)floorf((z - T.origin[2]) / T.cellSize[2]);
