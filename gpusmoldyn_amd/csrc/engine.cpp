@@ -328,11 +328,20 @@ smd_status Engine::finalize(uint32_t capacity) {
   HIPCHK(hipMemcpy(dN, &tot32, sizeof(uint32_t), hipMemcpyHostToDevice));
   HIPCHK(dalloc(&dFirst, first_words(nb)));
 
-  // Gaussian table.
+  // Gaussian table: the producer keeps its half t[1 .. kGaussDim / 2], from
+  // which the antisymmetry rebuilds every entry (gauss_entry).
   std::vector<float> g(kGaussDim);
   gauss_table(g.data(), kGaussDim);
-  HIPCHK(dalloc(&dGauss, kGaussDim));
-  HIPCHK(hipMemcpy(dGauss, g.data(), kGaussDim * sizeof(float), hipMemcpyHostToDevice));
+  for (int u = 0; u < kGaussDim; ++u) {
+    const int k = u - 1;
+    const float want = (u >= 1 && u <= kGaussDim / 2) ? g[k + 1] : -g[((kGaussDim - u) & (kGaussDim / 2 - 1)) + 1];
+    if (std::memcmp(&want, &g[u], sizeof(float)) != 0) {
+      set_error("Gaussian table is not antisymmetric");
+      return SMD_ERR_INVALID;
+    }
+  }
+  HIPCHK(dalloc(&dGauss, kGaussDim / 2));
+  HIPCHK(hipMemcpy(dGauss, g.data() + 1, kGaussDim / 2 * sizeof(float), hipMemcpyHostToDevice));
 
   // Random streams: stream k of a replica uses MersenneTwister.dat record
   // k % 32768 and seed k of that replica's seed sequence.

@@ -88,7 +88,7 @@ struct ProducerLds {
 template <int BS>
 __global__ __launch_bounds__(BS) void k_producer(ProducerArgs a, const ReplicaTables* __restrict__ tables) {
   __shared__ uint32_t lds_mt[kMtNN * BS];
-  __shared__ float lds_gauss[kGaussDim];
+  __shared__ float lds_gauss[kGaussDim / 2];  // half of the table (gauss_entry)
   __shared__ ProducerLds lds_tab[BS / 64];
   const uint32_t g = blockIdx.x * BS + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63u;
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(BS) void k_producer(ProducerArgs a, const ReplicaTa
   // a workgroup whose streams belong to no replica (the idle stream ranges of
   // a slot engine) leaves before it copies the Gaussian table
   if (!__syncthreads_or(r >= 0)) return;
-  for (int k = threadIdx.x; k < kGaussDim; k += BS) lds_gauss[k] = a.gauss[k];
+  for (int k = threadIdx.x; k < kGaussDim / 2; k += BS) lds_gauss[k] = a.gauss[k];
 
   const int rw = __builtin_amdgcn_readfirstlane(r);  // a wave never spans two replicas
   ProducerLds& L = lds_tab[threadIdx.x >> 6];
@@ -162,9 +162,9 @@ __global__ __launch_bounds__(BS) void k_producer(ProducerArgs a, const ReplicaTa
         mt_draw3<BS>(m, u0, u1, u2);
         const float rt = L.rate[t > 0 ? t : 0];
         float np[3];
-        np[0] = fmaf(rt, lds_gauss[u0 & (kGaussDim - 1)], p.x);
-        np[1] = fmaf(rt, lds_gauss[u1 & (kGaussDim - 1)], p.y);
-        np[2] = fmaf(rt, lds_gauss[u2 & (kGaussDim - 1)], p.z);
+        np[0] = fmaf(rt, gauss_entry(lds_gauss, u0 & (kGaussDim - 1)), p.x);
+        np[1] = fmaf(rt, gauss_entry(lds_gauss, u1 & (kGaussDim - 1)), p.y);
+        np[2] = fmaf(rt, gauss_entry(lds_gauss, u2 & (kGaussDim - 1)), p.z);
         bool wZero = false;  // the fourth component becomes 0 (offset products)
         int bt = 0;
         float bp[3] = {np[0], np[1], np[2]};

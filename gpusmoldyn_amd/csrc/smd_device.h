@@ -333,6 +333,18 @@ __device__ __forceinline__ int pack_index_type(uint32_t index, int type) {
 
 __device__ __forceinline__ uint32_t unpack_index(float w) { return __float_as_uint(w) >> kIndexShift; }
 
+// The Gaussian table t (kGaussDim entries, stats.cpp:202-232) is
+// antisymmetric and then rotated right by one: t[u] = -t[kGaussDim + 1 - u]
+// for 2 <= u < kGaussDim and t[0] = -t[1].  Its half h[j] = t[j + 1],
+// j < kGaussDim / 2, holds all of it: t[u] = h[u - 1] for 1 <= u <=
+// kGaussDim / 2, otherwise -h[(kGaussDim - u) mod kGaussDim / 2].
+__device__ __forceinline__ float gauss_entry(const float* h, uint32_t u) {
+  const uint32_t k = u - 1u;  // wraps for u = 0
+  const bool lower = k < (uint32_t)(kGaussDim / 2);
+  const float v = h[lower ? k : (((uint32_t)kGaussDim - u) & (uint32_t)(kGaussDim / 2 - 1))];
+  return lower ? v : -v;
+}
+
 // Number of set bits of m at lanes below this one (v_mbcnt).
 __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));

@@ -15,7 +15,7 @@ struct ProducerArgs {
   uint32_t* mtState;          // [20][Stot]
   const uint32_t* mtPar;      // [3][Stot]
   uint32_t Stot;
-  const float* gauss;         // 4096-entry table
+  const float* gauss;         // entries 1 to 2048 of the 4096-entry table (gauss_entry)
   float4* pt;                 // canonical (x, y, z, type bits), updated in place; also the pair-search source
   float* w;                   // canonical fourth position component (set to 0 for offset products)
   float4* addedPos;

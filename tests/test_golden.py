@@ -75,6 +75,21 @@ def test_gauss_table_closed_form():
     assert abs(t.max() - 3.6683607) < 1e-6
 
 
+def test_gauss_table_half_rebuilds_all():
+    # the producer keeps entries 1..2048 and rebuilds the rest by the
+    # antisymmetry of the rotated table (gauss_entry): t[u] = h[u - 1] for
+    # 1 <= u <= 2048, else -h[(4096 - u) mod 2048]; bit for bit on the golden table
+    t = np.load(os.path.join(GOLD, "gauss_table.npy")).astype(np.float32)
+    n = len(t)
+    h = t[1 : n // 2 + 1]
+    u = np.arange(n, dtype=np.uint32)
+    k = u - np.uint32(1)
+    lower = k < n // 2
+    v = h[np.where(lower, k, (np.uint32(n) - u) & np.uint32(n // 2 - 1)).astype(np.int64)]
+    rebuilt = np.where(lower, v, -v).astype(np.float32)
+    assert np.array_equal(rebuilt.view(np.uint32), t.view(np.uint32))
+
+
 def test_rxnparam_bitexact(smd):
     f = np.load(os.path.join(GOLD, "rxnparam.npz"))
     for s, a, b, v in f["numrxnrate"]:
