@@ -600,17 +600,23 @@ smd_status Engine::step_graph(int pairs) {
 // either way), else over all keys at once.
 bool Engine::seg_sorted() const { return segHigh && segSort && sb.algo == 1; }
 
-hipError_t Engine::sort_keys(uint32_t maxN) {
+SegSortArgs Engine::seg_args() const {
+  SegSortArgs g;
+  g.R = (int)R;
+  g.segOff = dRepOff;
+  g.segCount = dRepCount;
+  g.bucketOff = dBucketStart;
+  g.deadKey = deadKey;
+  g.lowBits = segLow;
+  g.highBits = segHigh;
+  g.scratch = dSegScratch;
+  return g;
+}
+
+hipError_t Engine::sort_keys(uint32_t maxN, bool tilesLaid) {
   if (seg_sorted()) {
-    SegSortArgs g;
-    g.R = (int)R;
-    g.segOff = dRepOff;
-    g.segCount = dRepCount;
-    g.bucketOff = dBucketStart;
-    g.deadKey = deadKey;
-    g.lowBits = segLow;
-    g.highBits = segHigh;
-    g.scratch = dSegScratch;
+    SegSortArgs g = seg_args();
+    g.tilesLaid = tilesLaid ? 1 : 0;
     return launch_sort_seg(sb, g, dN, maxN, true, stream);
   }
   return launch_sort(sb, dN, maxN, keyBits, radixBits, true, stream);
@@ -641,6 +647,13 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
   pa.tileAlive = dTileAlive;
   pa.tileBirth = dTileBirth;
   pa.retire = dRetire;
+  // the segmented sort's tile table by one more workgroup of the producer's
+  // launch (it reads only the per-replica counts the step starts with)
+  const bool tilesLaid = anyBim && seg_sorted();
+  if (tilesLaid) {
+    pa.segTilesOn = 1;
+    pa.segTiles = seg_tiles_args(seg_args(), dN, maxN, sb.status);
+  }
   HIPCHK(launch_producer(pa, producerBlock, stream));
   if (anyZero) {
     ZeroArgs za;
@@ -663,7 +676,7 @@ smd_status Engine::enqueue_kernels(uint32_t maxN, StageTimer* timer) {
     // the sort's last pass puts each particle's index into its sorted record
     // when the gather is fused into it and every index fits
     sb.packIndex = (sb.algo == 1 && cap <= kMaxPackedIndex) ? 1 : 0;
-    HIPCHK(sort_keys(maxN));
+    HIPCHK(sort_keys(maxN, tilesLaid));
     mark(2);
     HIPCHK(launch_first(sb.keys, dN, maxN, nb, dFirst, dLive, dTables, slotsMode ? slotBits : 0u, R, stream));
     mark(3);

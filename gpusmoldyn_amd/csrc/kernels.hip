@@ -82,6 +82,11 @@ struct ProducerLds {
   int32_t prodY[kMaxRxn];
 };
 
+// The segmented sort's tile table by one workgroup of BS threads (defined
+// with k_seg_tiles below); wsum holds BS / 64 words, sLong one.
+template <int BS>
+__device__ __forceinline__ void seg_tiles_body(const SegTilesArgs& a, uint32_t* wsum, uint32_t* sLong);
+
 // The replica tables arrive as a separate read-only, non-aliased argument:
 // their wave-uniform fields then load through the scalar cache, whose waits
 // do not hold up the particle records in flight.
@@ -90,6 +95,13 @@ __global__ __launch_bounds__(BS) void k_producer(ProducerArgs a, const ReplicaTa
   __shared__ uint32_t lds_mt[kMtNN * BS];
   __shared__ float lds_gauss[kGaussDim / 2];  // half of the table (gauss_entry)
   __shared__ ProducerLds lds_tab[BS / 64];
+  if (a.segTilesOn && blockIdx.x == gridDim.x - 1u) {
+    // the launch's extra workgroup: the segmented sort's tile table, which
+    // needs only the per-replica counts this step starts with (no stream's
+    // work depends on it, nor it on theirs)
+    seg_tiles_body<BS>(a.segTiles, lds_mt, lds_mt + BS / 64);
+    return;
+  }
   const uint32_t g = blockIdx.x * BS + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63u;
   const int r = (g < a.Stot) ? a.streamReplica[g] : -1;
@@ -1657,55 +1669,80 @@ __device__ __forceinline__ uint32_t seg_local(uint32_t key, uint32_t bucketOff, 
 // the per-segment removed counts.  Tiles stay inside the particle count and the table inside its
 // capacity; a segment layout that disagrees with either sets the sort-layout
 // status bit.
-__global__ __launch_bounds__(1024) void k_seg_tiles(const uint32_t* __restrict__ segOff,
-                                                    const uint32_t* __restrict__ segCount, int R, uint32_t tileSize,
-                                                    uint32_t maxTiles, uint2* __restrict__ tpos,
-                                                    uint32_t* __restrict__ tseg, uint32_t* __restrict__ nTiles,
-                                                    uint32_t* __restrict__ segFirst, uint32_t* __restrict__ nLong,
-                                                    uint32_t* __restrict__ repDead, const uint32_t* __restrict__ dN,
-                                                    uint32_t* __restrict__ status) {
-  __shared__ uint32_t wsum[1][16];
-  __shared__ uint32_t s_long;
-  if (threadIdx.x == 0) s_long = 0;
+// Exclusive scan of one value per thread over BS threads (BS / 64 waves,
+// wsum one word per wave); total gets the sum of all of them.
+template <int BS>
+__device__ __forceinline__ uint32_t block_exscan_bs(uint32_t v, uint32_t* wsum, uint32_t& total) {
+  constexpr uint32_t NW = BS / 64;
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
   __syncthreads();
-  const uint32_t nAll = *dN;
+  uint32_t pre = 0, all = 0;
+#pragma unroll
+  for (uint32_t q = 0; q < NW; ++q) {
+    const uint32_t ws = wsum[q];
+    pre += q < w ? ws : 0u;
+    all += ws;
+  }
+  __syncthreads();
+  total = all;
+  return pre + x - v;
+}
+
+template <int BS>
+__device__ __forceinline__ void seg_tiles_body(const SegTilesArgs& a, uint32_t* wsum, uint32_t* sLong) {
+  if (threadIdx.x == 0) *sLong = 0;
+  __syncthreads();
+  const uint32_t nAll = *a.dN;
   bool bad = false;
   uint32_t carry = 0;
-  for (uint32_t b = 0; b < (uint32_t)R; b += 1024) {
+  for (uint32_t b = 0; b < (uint32_t)a.R; b += BS) {
     const uint32_t r = b + threadIdx.x;
-    const uint32_t cnt = r < (uint32_t)R ? segCount[r] : 0u;
-    const uint32_t v[1] = {(cnt + tileSize - 1u) / tileSize};
-    uint32_t ex[1], tot[1];
-    block_exscan1024_k<1>(v, ex, tot, wsum);
-    if (r < (uint32_t)R) {
-      repDead[r] = 0u;
-      segFirst[r] = carry + ex[0];
-      if (v[0] > kSegScanLong) atomicAdd(&s_long, 1u);
-      const uint32_t off = segOff[r];
-      for (uint32_t k = 0; k < v[0]; ++k) {
-        const uint32_t idx = carry + ex[0] + k;
-        const uint32_t start = off + k * tileSize;
-        uint32_t len = min(tileSize, cnt - k * tileSize);
+    const uint32_t cnt = r < (uint32_t)a.R ? a.segCount[r] : 0u;
+    const uint32_t v = (cnt + a.tileSize - 1u) / a.tileSize;
+    uint32_t tot;
+    const uint32_t ex = block_exscan_bs<BS>(v, wsum, tot);
+    if (r < (uint32_t)a.R) {
+      a.repDead[r] = 0u;
+      a.segFirst[r] = carry + ex;
+      if (v > kSegScanLong) atomicAdd(sLong, 1u);
+      const uint32_t off = a.segOff[r];
+      for (uint32_t k = 0; k < v; ++k) {
+        const uint32_t idx = carry + ex + k;
+        const uint32_t start = off + k * a.tileSize;
+        uint32_t len = min(a.tileSize, cnt - k * a.tileSize);
         if (start >= nAll || len > nAll - start) {
           bad = true;
           len = start < nAll ? nAll - start : 0u;
         }
-        if (idx < maxTiles) {
-          tpos[idx] = make_uint2(start, len);
-          tseg[idx] = r | (k == 0 ? kSegHead : 0u);
+        if (idx < a.maxTiles) {
+          a.tpos[idx] = make_uint2(start, len);
+          a.tseg[idx] = r | (k == 0 ? kSegHead : 0u);
         } else {
           bad = true;
         }
       }
     }
-    carry += tot[0];
+    carry += tot;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    *nTiles = min(carry, maxTiles);
-    *nLong = s_long;
+    *a.nTiles = min(carry, a.maxTiles);
+    *a.nLong = *sLong;
   }
-  if (bad && status) atomicOr(status, kStatusSortLayout);
+  if (bad && a.status) atomicOr(a.status, kStatusSortLayout);
+}
+
+__global__ __launch_bounds__(1024) void k_seg_tiles(SegTilesArgs a) {
+  __shared__ uint32_t wsum[16];
+  __shared__ uint32_t s_long;
+  seg_tiles_body<1024>(a, wsum, &s_long);
 }
 
 template <int BITS, int ITEMS, bool COUNT_DEAD>
@@ -2069,13 +2106,14 @@ hipError_t launch_mt_fill(uint32_t* st, const uint32_t* par, uint32_t S, uint32_
 
 hipError_t launch_producer(const ProducerArgs& a, int blockSize, hipStream_t stream) {
   // the stream of global id g is g's in every shape: the results do not
-  // depend on the block size
+  // depend on the block size; one more workgroup for the tile table
+  const unsigned extra = a.segTilesOn ? 1u : 0u;
   if (blockSize == 512)
-    hipLaunchKernelGGL(k_producer<512>, dim3(cdiv(a.Stot, 512)), dim3(512), 0, stream, a, a.tables);
+    hipLaunchKernelGGL(k_producer<512>, dim3(cdiv(a.Stot, 512) + extra), dim3(512), 0, stream, a, a.tables);
   else if (blockSize == 128)
-    hipLaunchKernelGGL(k_producer<128>, dim3(cdiv(a.Stot, 128)), dim3(128), 0, stream, a, a.tables);
+    hipLaunchKernelGGL(k_producer<128>, dim3(cdiv(a.Stot, 128) + extra), dim3(128), 0, stream, a, a.tables);
   else
-    hipLaunchKernelGGL(k_producer<256>, dim3(cdiv(a.Stot, 256)), dim3(256), 0, stream, a, a.tables);
+    hipLaunchKernelGGL(k_producer<256>, dim3(cdiv(a.Stot, 256) + extra), dim3(256), 0, stream, a, a.tables);
   return hipGetLastError();
 }
 
@@ -2204,9 +2242,8 @@ static void seg_pass(SortBuffers& b, const SegSortArgs& g, const SegScratch& s, 
                        s.segTot, s.liveStart, s.deadStart, s.nLive, shift, idv, nullptr, nullptr, dN, b.status, 0);
 }
 
-template <int LB, int HB>
-static hipError_t seg_sort_run(SortBuffers& b, const SegSortArgs& g, const uint32_t* dN, uint32_t maxN,
-                               bool identityVals, hipStream_t stream) {
+// Where each table lives in g.scratch (seg_sort_words words).
+static SegScratch seg_scratch(const SegSortArgs& g, uint32_t maxN) {
   SegScratch s;
   s.maxTiles = seg_max_tiles(maxN, g.R);
   uint32_t* w = g.scratch;
@@ -2218,17 +2255,43 @@ static hipError_t seg_sort_run(SortBuffers& b, const SegSortArgs& g, const uint3
   s.nLive = w + 1;
   s.nLong = w + 2;
   w += 64;
-  constexpr int MB = LB > HB ? LB : HB;
+  const int mb = g.lowBits > g.highBits ? g.lowBits : g.highBits;
   s.counts = w;
-  w += ((size_t)1 << MB) * (s.maxTiles + 1);
+  w += ((size_t)1 << mb) * (s.maxTiles + 1);
   s.segTot = w;
-  w += (size_t)g.R << MB;
+  w += (size_t)g.R << mb;
   s.repDead = w;
   s.liveStart = w + g.R;
   s.deadStart = w + 2 * (size_t)g.R;
   s.segFirst = w + 3 * (size_t)g.R;
-  hipLaunchKernelGGL(k_seg_tiles, dim3(1), dim3(1024), 0, stream, g.segOff, g.segCount, g.R, 256u * kSegItems,
-                     s.maxTiles, s.tpos, s.tseg, s.nTiles, s.segFirst, s.nLong, s.repDead, dN, b.status);
+  return s;
+}
+
+SegTilesArgs seg_tiles_args(const SegSortArgs& g, const uint32_t* dN, uint32_t maxN, uint32_t* status) {
+  const SegScratch s = seg_scratch(g, maxN);
+  SegTilesArgs t;
+  t.segOff = g.segOff;
+  t.segCount = g.segCount;
+  t.R = g.R;
+  t.tileSize = 256u * kSegItems;
+  t.maxTiles = s.maxTiles;
+  t.tpos = s.tpos;
+  t.tseg = s.tseg;
+  t.nTiles = s.nTiles;
+  t.segFirst = s.segFirst;
+  t.nLong = s.nLong;
+  t.repDead = s.repDead;
+  t.dN = dN;
+  t.status = status;
+  return t;
+}
+
+template <int LB, int HB>
+static hipError_t seg_sort_run(SortBuffers& b, const SegSortArgs& g, const uint32_t* dN, uint32_t maxN,
+                               bool identityVals, hipStream_t stream) {
+  const SegScratch s = seg_scratch(g, maxN);
+  if (!g.tilesLaid)
+    hipLaunchKernelGGL(k_seg_tiles, dim3(1), dim3(1024), 0, stream, seg_tiles_args(g, dN, maxN, b.status));
   seg_pass<LB, true, false>(b, g, s, dN, 0, identityVals, stream);
   std::swap(b.keys, b.keysAlt);
   std::swap(b.vals, b.valsAlt);

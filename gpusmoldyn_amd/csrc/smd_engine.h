@@ -131,7 +131,8 @@ class Engine {
 
  private:
   smd_status enqueue_kernels(uint32_t maxN, StageTimer* timer);
-  hipError_t sort_keys(uint32_t maxN);
+  SegSortArgs seg_args() const;
+  hipError_t sort_keys(uint32_t maxN, bool tilesLaid = false);
   smd_status step_graph(int pairs);
   smd_status alloc_particle_buffers(uint32_t c);
   void free_particle_buffers();

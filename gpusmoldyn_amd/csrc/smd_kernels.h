@@ -7,6 +7,24 @@
 
 namespace smd {
 
+// Tile table of the segmented sort (k_seg_tiles): tiles of tileSize keys
+// inside each replica's segment, laid out from the per-replica counts.
+struct SegTilesArgs {
+  const uint32_t* segOff;
+  const uint32_t* segCount;
+  int R;
+  uint32_t tileSize;
+  uint32_t maxTiles;
+  uint2* tpos;
+  uint32_t* tseg;
+  uint32_t* nTiles;
+  uint32_t* segFirst;
+  uint32_t* nLong;
+  uint32_t* repDead;
+  const uint32_t* dN;
+  uint32_t* status;
+};
+
 struct ProducerArgs {
   const ReplicaTables* tables;
   const int* streamReplica;   // per global stream: replica id (padding lanes of a replica included), -1 none
@@ -27,6 +45,10 @@ struct ProducerArgs {
   uint32_t* tileAlive;
   uint32_t* tileBirth;
   const uint32_t* retire;     // per replica: remove every particle this step (nullptr: none)
+  // 1: one extra workgroup of the launch lays out the segmented sort's tiles
+  // (it reads only the per-replica counts the step starts with)
+  int segTilesOn = 0;
+  SegTilesArgs segTiles;
 };
 
 struct SortBuffers {
@@ -61,6 +83,7 @@ struct SegSortArgs {
   int lowBits;         // first-pass digit width (7 to 10 bits)
   int highBits;        // second-pass digit width (7 to 10 bits)
   uint32_t* scratch;   // seg_sort_words(maxN, R, max(lowBits, highBits)) words
+  int tilesLaid = 0;   // 1: the tile table is already laid out (seg_tiles_args, by the producer launch)
 };
 
 // Digit widths of the segmented sort for replica grids of at most 2^maxGridBits
@@ -188,6 +211,9 @@ hipError_t launch_sort(SortBuffers& b, const uint32_t* dN, uint32_t maxN, uint32
 size_t seg_sort_words(uint32_t maxN, int R, int maxBits);
 hipError_t launch_sort_seg(SortBuffers& b, const SegSortArgs& g, const uint32_t* dN, uint32_t maxN,
                            bool identityVals, hipStream_t stream);
+// The tile-table arguments of a segmented sort of at most maxN keys (the
+// addresses inside g.scratch that launch_sort_seg uses).
+SegTilesArgs seg_tiles_args(const SegSortArgs& g, const uint32_t* dN, uint32_t maxN, uint32_t* status);
 // Counter words the count / scan / scatter sort (SortBuffers::algo = 1)
 // uses for maxN keys: per-tile digit counts and the digit totals.
 size_t sort3k_counts_words(uint32_t maxN, uint32_t sizeHint, int radixBits);
